@@ -1,0 +1,178 @@
+"""Throughput benchmark of the MI355X Fenton-Karma forward-Euler path.
+
+One bench "step" = one full forward solve of the single-large-tissue
+workload: 1024 x 1024 grid, PARAMSET_3, scarred diffusivity, S1 (WEST stripe
+at step 0) + S2 (NORTH half field at a seeded step in [28k, 36k)), 1e5 Euler
+steps at dt = dx = 0.01, inputs resident in HBM. Metric: giga cell-updates per
+second (GCUPS = H * W * Euler steps / s / 1e9).
+
+Prints ONE JSON line (rank 0). Progress goes to stderr.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--sim-steps S] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+BYTES_PER_CELL_UPDATE = 28  # read v, w, u, D + write v, w, u (fp32)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(sim, target_s=12.0):
+    """fp32 C restatement of the reference op graph (oracle/fk_oracle.c) on
+    the host cores, over a bounded sample of the same workload."""
+    import numpy as np
+
+    from oracle import c_oracle, fk_oracle
+
+    stim = [fk_oracle.Stimulus(s.protocol, s.field) for s in sim.stimuli]
+    st = fk_oracle.init(sim.shape, np.float32)
+    cores = cpu_cores()
+    H, W = sim.shape
+    n0 = 10
+    t0 = time.perf_counter()
+    st = c_oracle.forward_euler(st, 0, n0, sim.params, sim.D, stim, sim.dt, sim.dx,
+                                np.float32, cores)
+    dt0 = time.perf_counter() - t0
+    n1 = int(max(10, min(5000, n0 * target_s / max(dt0, 1e-6))))
+    t0 = time.perf_counter()
+    c_oracle.forward_euler(st, n0, n0 + n1, sim.params, sim.D, stim, sim.dt, sim.dx,
+                           np.float32, cores)
+    el = time.perf_counter() - t0
+    return {"value": H * W * n1 / el / 1e9, "unit": "GCUPS", "cores": cores, "kind": "port",
+            "sample": "%dx%d grid, Euler steps %d..%d of the same inputs, fp32 C "
+                      "restatement of the reference op graph, %d OpenMP threads, %.1f s"
+                      % (H, W, n0, n0 + n1, cores, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sim-steps", type=int, default=100_000)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain", action="store_true",
+                    help="also time the plain one-thread-per-cell kernel")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the solver has no multi-GPU path in this "
+              "build (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from cardiax_amd import solve, workloads
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    H = W = args.size
+    sim = workloads.c2_inputs(args.seed, shape=(H, W), steps=args.sim_steps)
+    stims = [(s.protocol, torch.from_numpy(s.field).to(dev)) for s in sim.stimuli]
+    D = torch.from_numpy(sim.D).to(dev)
+    st0 = solve.init((H, W))
+    torch.cuda.synchronize()
+
+    def run():
+        return solve._forward_euler(st0, sim.t0, sim.t1, sim.params, D, stims, sim.dt, sim.dx)
+
+    for i in range(args.warmup):
+        out = run()
+        torch.cuda.synchronize()
+        log("warmup %d/%d done" % (i + 1, args.warmup))
+
+    # Timed region: K full solves; HIP events on the stream the kernels use.
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(args.steps):
+        out = run()
+        log("step %d/%d enqueued" % (i + 1, args.steps))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ev0.elapsed_time(ev1)
+    finite = bool(torch.isfinite(out.u).all().item())
+    umax = float(out.u.abs().max().item())
+    cell_updates = H * W * sim.t1 * args.steps
+    gcups = cell_updates / wall / 1e9
+
+    # Dominant kernel: one launch of the tiled step kernel per Euler step.
+    launches = sim.t1 * args.steps
+    per_launch_s = (ev_ms / 1e3) / launches
+    achieved = BYTES_PER_CELL_UPDATE * H * W / per_launch_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_tile<32,true>", "per_launch_us": round(per_launch_s * 1e6, 3),
+                "bytes_per_launch": BYTES_PER_CELL_UPDATE * H * W}
+
+    extra = {"event_gcups": round(cell_updates / (ev_ms / 1e3) / 1e9, 3),
+             "u_finite": finite, "u_absmax": round(umax, 4)}
+    if args.plain:
+        n = 2000
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        solve._forward_euler_plain(st0, 0, n, sim.params, D, stims, sim.dt, sim.dx)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        extra["plain_kernel_gcups"] = round(H * W * n / (e0.elapsed_time(e1) / 1e3) / 1e9, 3)
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(sim)
+        log("cpu baseline: %.4f GCUPS on %d cores" % (cpu["value"], cpu["cores"]))
+
+    rec = {
+        "metric": "GCUPS (giga cell-updates/s), fused forward-Euler FK step, fp32",
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scar D-map and S1/S2 stimuli)",
+        "config": {"workload": "single large tissue: %dx%d grid, %d Euler steps per bench "
+                               "step, PARAMSET_3, scarred D, S1 WEST + S2 NORTH at step %d"
+                               % (H, W, sim.t1, sim.meta["s2_start"]),
+                   "H": H, "W": W, "euler_steps": sim.t1, "dt": sim.dt, "dx": sim.dx,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

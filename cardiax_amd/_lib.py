@@ -1,0 +1,110 @@
+"""ctypes binding of ``libcardiax_amd.so`` (declared in include/cardiax_amd.h).
+
+torch is imported first so that the library's ``libamdhip64.so.7`` dependency
+resolves to the HIP runtime torch already loaded: one runtime per process.
+Loading fails loudly when the library has not been built; there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the CDLL below)
+
+from .params import Params
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcardiax_amd.so")
+
+FK_OK = 0
+FK_ERR_ARG = -1
+FK_ERR_HIP = -2
+FK_ERR_UNSUPPORTED = -3
+FK_MAX_STIMULI = 32
+
+c_f = ctypes.c_float
+c_d = ctypes.c_double
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_sz = ctypes.c_size_t
+c_vp = ctypes.c_void_p
+
+
+class FkParams(ctypes.Structure):
+    """struct fk_params -- the 14 Params fields in order (cardiax/params.py:4-18)."""
+    _fields_ = [(name, c_f) for name in Params._fields]
+
+
+class FkStimulus(ctypes.Structure):
+    """struct fk_stimulus (cardiax/stimulus.py:13-21)."""
+    _fields_ = [("start", c_d), ("duration", c_d), ("period", c_d), ("field", c_vp)]
+
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "fk_last_error": (ctypes.c_char_p, []),
+    "fk_abi_version": (c_i32, []),
+    "fk_init": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "fk_step": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
+                        ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_vp, c_vp, c_vp, c_vp]),
+    "fk_step_euler": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
+                              ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_f, c_vp, c_vp,
+                              c_vp, c_vp]),
+    "fk_forward_workspace_bytes": (c_sz, [c_i32, c_i32]),
+    "fk_forward_euler": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32,
+                                 c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d, c_d, c_f, c_f,
+                                 c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "fk_forward_euler_plain": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp,
+                                       c_i32, c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d,
+                                       c_d, c_f, c_f, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "fk_gradient": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp]),
+    "fk_stimulate": (c_i32, [c_d, c_vp, c_vp, c_i64, ctypes.POINTER(FkStimulus), c_i32, c_vp]),
+}
+
+_lib = None
+
+
+def exported_symbols():
+    """Names of every entry point this binding declares."""
+    return sorted(_SIGNATURES)
+
+
+def lib() -> ctypes.CDLL:
+    """Load (once) and return the native library; RuntimeError if missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "native library %s is missing: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    """Raise the Python error matching a nonzero C-ABI return code."""
+    if rc == FK_OK:
+        return
+    msg = lib().fk_last_error().decode("utf-8", "replace")
+    if rc == FK_ERR_ARG:
+        raise ValueError(msg)
+    if rc == FK_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError("HIP error: " + msg)
+
+
+def make_params(params) -> FkParams:
+    vals = tuple(params)
+    if len(vals) != 14:
+        raise TypeError("params must have the 14 Params fields, got %d values" % len(vals))
+    return FkParams(*[float(x) for x in vals])
+
+
+def ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()

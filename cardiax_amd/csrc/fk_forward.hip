@@ -1,0 +1,417 @@
+// Forward kernels of the Fenton-Karma step and the C-ABI entry points of the
+// single-simulation path (include/cardiax_amd.h).
+//
+// Two kernels compute the same step (cardiax/solve.py:26-70):
+//   k_plain  one thread per cell, every neighbour read from global memory with
+//            clamped indices (the edge padding), no LDS. The unoptimised
+//            baseline and a structurally independent cross-check.
+//   k_tile   64 x TH output tile per 256-thread workgroup. u (halo 4) and D
+//            (halo 2) are staged in LDS once; the first derivatives u_x, u_y
+//            are formed once per cell into LDS and reused by the 5-point second
+//            derivative; v, w are read straight from HBM into registers. One
+//            HBM pass per step: read v, w, u, D, write v, w, u.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "cardiax_amd.h"
+#include "fk_device.h"
+#include "fk_host.h"
+
+namespace fk {
+
+// ---------------------------------------------------------------------------
+// Plain kernel
+// ---------------------------------------------------------------------------
+template <bool EULER>
+__global__ void __launch_bounds__(256)
+k_plain(Coef c, const float* __restrict__ D, const float* __restrict__ v,
+        const float* __restrict__ w, const float* __restrict__ u, int H, int W, StimArgs sa,
+        float dt, float inv_dx2, float* __restrict__ o_v, float* __restrict__ o_w,
+        float* __restrict__ o_u) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (i >= H || j >= W) return;
+  const size_t idx = (size_t)i * W + j;
+
+  auto ux = [&](int r) { return u[(size_t)clampi(r, 0, H - 1) * W + j]; };
+  auto uy = [&](int s) { return u[(size_t)i * W + clampi(s, 0, W - 1)]; };
+  auto Dx = [&](int r) { return D[(size_t)clampi(r, 0, H - 1) * W + j]; };
+  auto Dy = [&](int s) { return D[(size_t)i * W + clampi(s, 0, W - 1)]; };
+  auto gx = [&](int q) { return d1_general(ux, q, H); };
+  auto gy = [&](int q) { return d1_general(uy, q, W); };
+
+  const float uxx = d2_from_d1(gx, i, H);
+  const float uyy = d2_from_d1(gy, j, W);
+  const float dDx = d1_general(Dx, i, H);
+  const float dDy = d1_general(Dy, j, W);
+  const float Dc = D[idx];
+  const float del_u = (Dc * (uxx + uyy) + dDx * gx(i) + dDy * gy(j)) * inv_dx2;
+
+  const float vc = v[idx], wc = w[idx], uc = u[idx];
+  float dv, dw, jion;
+  reaction(c, vc, wc, uc, dv, dw, jion);
+  jion = stimulate_cell(sa, idx, jion);
+  const float du = del_u + jion;
+  if (EULER) {
+    o_v[idx] = vc + dt * dv;
+    o_w[idx] = wc + dt * dw;
+    o_u[idx] = uc + dt * du;
+  } else {
+    o_v[idx] = dv;
+    o_w[idx] = dw;
+    o_u[idx] = du;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled kernel
+// ---------------------------------------------------------------------------
+constexpr int TW = 64;  // tile width = one wavefront of columns
+
+// Tile origin along an axis of N cells: tiles of T cells, the last one shifted
+// back to end at N (overlapping its neighbour, which then writes identical
+// values twice) so that every cell at index N-1 sits >= 3 rows inside its tile
+// and the one-sided rules only read inside the staged halo.
+__device__ __forceinline__ int tile_origin(int b, int T, int N) {
+  const int o = b * T;
+  return (N >= T && o > N - T) ? N - T : o;
+}
+
+template <int TH, bool EULER>
+__global__ void __launch_bounds__(256)
+k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
+       const float* __restrict__ w, const float* __restrict__ u, int H, int W, StimArgs sa,
+       float dt, float inv_dx2, float* __restrict__ o_v, float* __restrict__ o_w,
+       float* __restrict__ o_u) {
+  constexpr int UR = TH + 8, UC = TW + 8;  // u, halo 4
+  constexpr int DR = TH + 4, DC = TW + 4;  // D, halo 2
+  __shared__ float s_u[UR][UC];
+  __shared__ float s_D[DR][DC];
+  __shared__ float s_gx[TH + 4][TW];  // u_x at tile rows -2 .. TH+1
+  __shared__ float s_gy[TH][TW + 4];  // u_y at tile cols -2 .. TW+1
+
+  const int lane = threadIdx.x;  // column in tile
+  const int wv = threadIdx.y;    // 0..3
+  const int r0 = tile_origin(blockIdx.y, TH, H);
+  const int c0 = tile_origin(blockIdx.x, TW, W);
+
+  // Stage u and D with clamped indices: this *is* the edge padding.
+  for (int r = wv; r < UR; r += 4) {
+    const float* row = u + (size_t)clampi(r0 - 4 + r, 0, H - 1) * W;
+    s_u[r][lane] = row[clampi(c0 - 4 + lane, 0, W - 1)];
+    if (lane < 8) s_u[r][TW + lane] = row[clampi(c0 + TW - 4 + lane, 0, W - 1)];
+  }
+  for (int r = wv; r < DR; r += 4) {
+    const float* row = D + (size_t)clampi(r0 - 2 + r, 0, H - 1) * W;
+    s_D[r][lane] = row[clampi(c0 - 2 + lane, 0, W - 1)];
+    if (lane < 4) s_D[r][TW + lane] = row[clampi(c0 + TW - 2 + lane, 0, W - 1)];
+  }
+  __syncthreads();
+
+  // Every first/second derivative this tile needs is a central one.
+  const bool interior =
+      (r0 >= 3) && (r0 + TH + 3 <= H) && (c0 >= 3) && (c0 + TW + 3 <= W);
+
+  // u_x on tile rows lr in [-2, TH+2), column `lane`; u_y on rows [0, TH),
+  // tile columns lc in [-2, TW+2).
+  if (interior) {
+    for (int r = wv; r < TH + 4; r += 4) {
+      const int a = r + 2;  // s_u row of tile row lr = r - 2
+      s_gx[r][lane] = d1_interior(s_u[a - 2][lane + 4], s_u[a - 1][lane + 4],
+                                  s_u[a + 1][lane + 4], s_u[a + 2][lane + 4]);
+    }
+    for (int r = wv; r < TH; r += 4) {
+      const float* ur = s_u[r + 4];
+      {
+        const int b = lane + 2;  // s_u column of tile column lc = lane - 2
+        s_gy[r][lane] = d1_interior(ur[b - 2], ur[b - 1], ur[b + 1], ur[b + 2]);
+      }
+      if (lane < 4) {
+        const int b = TW + 2 + lane;
+        s_gy[r][TW + lane] = d1_interior(ur[b - 2], ur[b - 1], ur[b + 1], ur[b + 2]);
+      }
+    }
+  } else {
+    for (int r = wv; r < TH + 4; r += 4) {
+      const int q = r0 + r - 2;  // global row
+      auto f = [&](int gr) { return s_u[gr - r0 + 4][lane + 4]; };
+      s_gx[r][lane] = d1_general(f, q, H);
+    }
+    for (int r = wv; r < TH; r += 4) {
+      const float* ur = s_u[r + 4];
+      auto f = [&](int gc) { return ur[gc - c0 + 4]; };
+      s_gy[r][lane] = d1_general(f, c0 + lane - 2, W);
+      if (lane < 4) s_gy[r][TW + lane] = d1_general(f, c0 + TW - 2 + lane, W);
+    }
+  }
+  __syncthreads();
+
+  const int gj = c0 + lane;
+  for (int r = wv; r < TH; r += 4) {
+    const int gi = r0 + r;
+    float uxx, uyy, dDx, dDy;
+    if (interior) {
+      uxx = d1_interior(s_gx[r][lane], s_gx[r + 1][lane], s_gx[r + 3][lane],
+                        s_gx[r + 4][lane]);
+      uyy = d1_interior(s_gy[r][lane], s_gy[r][lane + 1], s_gy[r][lane + 3],
+                        s_gy[r][lane + 4]);
+      dDx = d1_interior(s_D[r][lane + 2], s_D[r + 1][lane + 2], s_D[r + 3][lane + 2],
+                        s_D[r + 4][lane + 2]);
+      dDy = d1_interior(s_D[r + 2][lane], s_D[r + 2][lane + 1], s_D[r + 2][lane + 3],
+                        s_D[r + 2][lane + 4]);
+    } else {
+      auto gx = [&](int q) { return s_gx[q - r0 + 2][lane]; };
+      auto gy = [&](int q) { return s_gy[r][q - c0 + 2]; };
+      auto Dx = [&](int q) { return s_D[q - r0 + 2][lane + 2]; };
+      auto Dy = [&](int q) { return s_D[r + 2][q - c0 + 2]; };
+      uxx = d2_from_d1(gx, gi, H);
+      uyy = d2_from_d1(gy, gj, W);
+      dDx = d1_general(Dx, gi, H);
+      dDy = d1_general(Dy, gj, W);
+    }
+    if (gi >= H || gj >= W) continue;
+    const float Dc = s_D[r + 2][lane + 2];
+    const float del_u =
+        (Dc * (uxx + uyy) + dDx * s_gx[r + 2][lane] + dDy * s_gy[r][lane + 2]) * inv_dx2;
+    const size_t idx = (size_t)gi * W + gj;
+    const float vc = v[idx], wc = w[idx], uc = s_u[r + 4][lane + 4];
+    float dv, dw, jion;
+    reaction(c, vc, wc, uc, dv, dw, jion);
+    jion = stimulate_cell(sa, idx, jion);
+    const float du = del_u + jion;
+    if (EULER) {
+      o_v[idx] = vc + dt * dv;
+      o_w[idx] = wc + dt * dw;
+      o_u[idx] = uc + dt * du;
+    } else {
+      o_v[idx] = dv;
+      o_w[idx] = dw;
+      o_u[idx] = du;
+    }
+  }
+}
+
+constexpr int kTileH = 32;
+
+// ---------------------------------------------------------------------------
+// Small kernels: init, gradient, stimulate
+// ---------------------------------------------------------------------------
+__global__ void k_init(float* v, float* w, float* u, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    v[i] = 1.0f;
+    w[i] = 1.0f;
+    u[i] = 0.0f;
+  }
+}
+
+// cardiax/solve.py:225-254 on an unpadded (rows, cols) array.
+__global__ void k_gradient(const float* __restrict__ a, float* __restrict__ out, int rows,
+                           int cols, int axis) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * 4 + threadIdx.y;
+  if (i >= rows || j >= cols) return;
+  const int n = axis == 0 ? rows : cols;
+  const int p = axis == 0 ? i : j;
+  auto f = [&](int k) {
+    return axis == 0 ? a[(size_t)k * cols + j] : a[(size_t)i * cols + k];
+  };
+  float g;
+  if (p <= 1)
+    g = d1_forward(f(p), f(p + 1), f(p + 2), f(p + 3));
+  else if (p >= n - 2)
+    g = d1_backward(f(p - 3), f(p - 2), f(p - 1), f(p));
+  else
+    g = d1_interior(f(p - 2), f(p - 1), f(p + 1), f(p + 2));
+  out[(size_t)i * cols + j] = g;
+}
+
+__global__ void k_stimulate(const float* __restrict__ X, float* __restrict__ out, int64_t n,
+                            StimArgs sa) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = stimulate_cell(sa, (size_t)i, X[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers
+// ---------------------------------------------------------------------------
+enum class Kernel { Plain, Tile };
+
+static hipError_t launch_step(Kernel kind, bool euler, const Coef& c, const float* D,
+                              const float* v, const float* w, const float* u, int H, int W,
+                              const StimArgs& sa, float dt, float inv_dx2, float* ov,
+                              float* ow, float* ou, hipStream_t s) {
+  const dim3 block(64, 4);
+  if (kind == Kernel::Plain) {
+    const dim3 grid((W + 63) / 64, (H + 3) / 4);
+    if (euler)
+      k_plain<true><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou);
+    else
+      k_plain<false><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou);
+  } else {
+    const dim3 grid((W + TW - 1) / TW, (H + kTileH - 1) / kTileH);
+    if (euler)
+      k_tile<kTileH, true><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov,
+                                                  ow, ou);
+    else
+      k_tile<kTileH, false><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov,
+                                                   ow, ou);
+  }
+  return hipGetLastError();
+}
+
+static int32_t forward_loop(Kernel kind, const fk_params* params, const float* D,
+                            const float* v_in, const float* w_in, const float* u_in, int32_t H,
+                            int32_t W, const fk_stimulus* stimuli, int32_t n_stimuli,
+                            double t, double t_end, float dt, float dx, float* v_out,
+                            float* w_out, float* u_out, void* work, size_t work_bytes,
+                            void* stream) {
+  ErrorScope es;
+  if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
+      !check_ptrs({D, v_in, w_in, u_in, v_out, w_out, u_out}) || !check_time(t, t_end) ||
+      !check_scalar("dt", dt) || !check_dx(dx))
+    return last_code();
+  const size_t n = (size_t)H * W;
+  const int64_t steps = loop_count(t, t_end);
+  if (steps >= 2 && (work == nullptr || work_bytes < fk_forward_workspace_bytes(H, W)))
+    return set_error(FK_ERR_ARG, "workspace too small: need %zu bytes, got %zu",
+                     fk_forward_workspace_bytes(H, W), work_bytes);
+  hipStream_t s = (hipStream_t)stream;
+  if (steps == 0) {
+    for (auto pr : {std::make_pair(v_in, v_out), std::make_pair(w_in, w_out),
+                    std::make_pair(u_in, u_out)})
+      HIP_CHECK(hipMemcpyAsync(pr.second, pr.first, n * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
+    return FK_OK;
+  }
+  const Coef c = make_coef(*params);
+  const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
+  float* wk = (float*)work;
+  float* bufs[2][3] = {{v_out, w_out, u_out}, {wk, wk + n, wk + 2 * n}};
+  // Ping-pong so that the last step lands in the output buffers.
+  int dst = (steps % 2 == 1) ? 0 : 1;
+  const float* src[3] = {v_in, w_in, u_in};
+  StimSchedule sched(stimuli, n_stimuli);
+  for (int64_t k = 0; k < steps; ++k) {
+    const StimArgs& sa = sched.at(t + (double)k);
+    float** d = bufs[dst];
+    HIP_CHECK(launch_step(kind, true, c, D, src[0], src[1], src[2], H, W, sa, dt, inv_dx2,
+                          d[0], d[1], d[2], s));
+    src[0] = d[0];
+    src[1] = d[1];
+    src[2] = d[2];
+    dst ^= 1;
+  }
+  return FK_OK;
+}
+
+}  // namespace fk
+
+using namespace fk;
+
+extern "C" {
+
+int32_t fk_abi_version(void) { return 1; }
+
+int32_t fk_init(float* v, float* w, float* u, int64_t n_cells, void* stream) {
+  ErrorScope es;
+  if (!check_ptrs({v, w, u}) || n_cells < 0)
+    return set_error(FK_ERR_ARG, "fk_init: null pointer or negative size");
+  if (n_cells == 0) return FK_OK;
+  const int blocks = (int)std::min<int64_t>((n_cells + 255) / 256, 4096);
+  k_init<<<blocks, 256, 0, (hipStream_t)stream>>>(v, w, u, n_cells);
+  HIP_CHECK(hipGetLastError());
+  return FK_OK;
+}
+
+int32_t fk_step(const fk_params* params, const float* D, const float* v, const float* w,
+                const float* u, int32_t H, int32_t W, const fk_stimulus* stimuli,
+                int32_t n_stimuli, double t, float dx, float* dv, float* dw, float* du,
+                void* stream) {
+  ErrorScope es;
+  if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
+      !check_ptrs({D, v, w, u, dv, dw, du}) || !check_scalar("t", t) || !check_dx(dx))
+    return last_code();
+  StimSchedule sched(stimuli, n_stimuli);
+  const Coef c = make_coef(*params);
+  const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
+  HIP_CHECK(launch_step(Kernel::Tile, false, c, D, v, w, u, H, W, sched.at(t), 0.0f, inv_dx2,
+                        dv, dw, du, (hipStream_t)stream));
+  return FK_OK;
+}
+
+int32_t fk_step_euler(const fk_params* params, const float* D, const float* v,
+                      const float* w, const float* u, int32_t H, int32_t W,
+                      const fk_stimulus* stimuli, int32_t n_stimuli, double t, float dt,
+                      float dx, float* v_out, float* w_out, float* u_out, void* stream) {
+  ErrorScope es;
+  if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
+      !check_ptrs({D, v, w, u, v_out, w_out, u_out}) || !check_scalar("t", t) ||
+      !check_scalar("dt", dt) || !check_dx(dx))
+    return last_code();
+  StimSchedule sched(stimuli, n_stimuli);
+  const Coef c = make_coef(*params);
+  const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
+  HIP_CHECK(launch_step(Kernel::Tile, true, c, D, v, w, u, H, W, sched.at(t), dt, inv_dx2,
+                        v_out, w_out, u_out, (hipStream_t)stream));
+  return FK_OK;
+}
+
+size_t fk_forward_workspace_bytes(int32_t H, int32_t W) {
+  if (H <= 0 || W <= 0) return 0;
+  return 3 * (size_t)H * (size_t)W * sizeof(float);
+}
+
+int32_t fk_forward_euler(const fk_params* params, const float* D, const float* v_in,
+                         const float* w_in, const float* u_in, int32_t H, int32_t W,
+                         const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                         double t_end, float dt, float dx, float* v_out, float* w_out,
+                         float* u_out, void* work, size_t work_bytes, void* stream) {
+  return forward_loop(Kernel::Tile, params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli, t,
+                      t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
+}
+
+int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const float* v_in,
+                               const float* w_in, const float* u_in, int32_t H, int32_t W,
+                               const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                               double t_end, float dt, float dx, float* v_out,
+                               float* w_out, float* u_out, void* work, size_t work_bytes,
+                               void* stream) {
+  return forward_loop(Kernel::Plain, params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli,
+                      t, t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
+}
+
+int32_t fk_gradient(const float* a, float* out, int32_t rows, int32_t cols, int32_t axis,
+                    void* stream) {
+  ErrorScope es;
+  if (!check_ptrs({a, out})) return FK_ERR_ARG;
+  if (axis != 0 && axis != 1) return set_error(FK_ERR_ARG, "axis must be 0 or 1, got %d", axis);
+  if (rows <= 0 || cols <= 0) return set_error(FK_ERR_ARG, "empty array %dx%d", rows, cols);
+  const int n = axis == 0 ? rows : cols;
+  if (n < 5)
+    return set_error(FK_ERR_ARG, "gradient needs >= 5 entries along axis %d, got %d", axis, n);
+  const dim3 block(64, 4), grid((cols + 63) / 64, (rows + 3) / 4);
+  k_gradient<<<grid, block, 0, (hipStream_t)stream>>>(a, out, rows, cols, axis);
+  HIP_CHECK(hipGetLastError());
+  return FK_OK;
+}
+
+int32_t fk_stimulate(double t, const float* X, float* out, int64_t n_cells,
+                     const fk_stimulus* stimuli, int32_t n_stimuli, void* stream) {
+  ErrorScope es;
+  if (n_cells < 0) return set_error(FK_ERR_ARG, "fk_stimulate: negative size");
+  if (!check_ptrs({X, out}) || !check_stimuli(stimuli, n_stimuli) || !check_scalar("t", t))
+    return last_code();
+  if (n_cells == 0) return FK_OK;
+  StimSchedule sched(stimuli, n_stimuli);
+  const int blocks = (int)std::min<int64_t>((n_cells + 255) / 256, 4096);
+  k_stimulate<<<blocks, 256, 0, (hipStream_t)stream>>>(X, out, n_cells, sched.at(t));
+  HIP_CHECK(hipGetLastError());
+  return FK_OK;
+}
+
+}  // extern "C"

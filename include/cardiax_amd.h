@@ -1,0 +1,122 @@
+/*
+ * cardiax_amd -- C-ABI of the MI355X (gfx950) Fenton-Karma forward-Euler solver.
+ *
+ * The reference (cardiax, pure Python/JAX) has no native interface; its plug
+ * point is the integrator callable of cardiax/solve.py:188,209-218,
+ *     f(state, t, t_end, params, diffusivity, stimuli, dt, dx) -> State,
+ * whose Euler implementation is _forward_euler (cardiax/solve.py:92-100).
+ * Each entry point below replaces one public function of cardiax/solve.py and
+ * is bound from Python with ctypes (cardiax_amd/_lib.py).
+ *
+ * Conventions
+ *   - Fields are float32, C-order row-major (H, W): axis 0 (rows) is the
+ *     reference's "x" (cardiax/solve.py:49), axis 1 (columns) its "y" (:50).
+ *   - All array pointers are device pointers; structs and descriptor arrays
+ *     are host memory read before the call returns.
+ *   - Work is enqueued on `stream` (a hipStream_t; NULL = the null stream) and
+ *     is asynchronous: no entry point synchronises the stream.
+ *   - No entry point allocates device memory except fk_batch_create, which
+ *     owns the device copy of the batch work list until fk_batch_destroy.
+ *   - Return value: FK_OK, or a negative FK_ERR_* code; fk_last_error()
+ *     returns a thread-local message for the last failing call.
+ */
+#ifndef CARDIAX_AMD_H
+#define CARDIAX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FK_OK = 0,
+  FK_ERR_ARG = -1,         /* invalid argument: shape, pointer, count, range */
+  FK_ERR_HIP = -2,         /* a HIP runtime call failed */
+  FK_ERR_UNSUPPORTED = -3  /* valid request outside this build's limits */
+};
+
+/* Most stimuli one call accepts (the reference takes any list length). */
+#define FK_MAX_STIMULI 32
+
+/* cardiax/params.py:4-18 -- the 14 Fenton-Karma constants in Params order. */
+typedef struct fk_params {
+  float tau_v_plus, tau_v1_minus, tau_v2_minus, tau_w_plus, tau_w_minus;
+  float tau_d, tau_0, tau_r, tau_si, k, V_csi, V_c, V_v, Cm;
+} fk_params;
+
+/* cardiax/stimulus.py:13-21 -- Protocol(start, duration, period), in steps,
+ * and the (H, W) float32 field. Activity at time t follows
+ * cardiax/solve.py:262-267: t >= start && mod(start - t + 1, period) < duration
+ * with a floor modulo; it is evaluated on the host in double precision, which
+ * equals the reference's float32/int32 evaluation for integral times < 2^24. */
+typedef struct fk_stimulus {
+  double start, duration, period;
+  const float* field;
+} fk_stimulus;
+
+/* Thread-local description of the last error; "" when none. */
+const char* fk_last_error(void);
+
+/* ABI revision of this library (incremented on incompatible change). */
+int32_t fk_abi_version(void);
+
+/* cardiax/solve.py:18-23 -- v = w = 1, u = 0 over n_cells cells. */
+int32_t fk_init(float* v, float* w, float* u, int64_t n_cells, void* stream);
+
+/* cardiax/solve.py:26-65 -- the time derivatives (d_v, d_w, d_u) of one
+ * state at time t: edge-padded fields, gates and ionic currents, stimulus
+ * replacement, 4th-order heterogeneous diffusion, crop. H, W >= 3. */
+int32_t fk_step(const fk_params* params, const float* D, const float* v, const float* w,
+                const float* u, int32_t H, int32_t W, const fk_stimulus* stimuli,
+                int32_t n_stimuli, double t, float dx, float* dv, float* dw, float* du,
+                void* stream);
+
+/* cardiax/solve.py:68-70 -- one explicit Euler step, out = in + dt * step(in).
+ * Outputs must not alias inputs. */
+int32_t fk_step_euler(const fk_params* params, const float* D, const float* v,
+                      const float* w, const float* u, int32_t H, int32_t W,
+                      const fk_stimulus* stimuli, int32_t n_stimuli, double t, float dt,
+                      float dx, float* v_out, float* w_out, float* u_out, void* stream);
+
+/* Scratch bytes fk_forward_euler needs for an H x W grid. */
+size_t fk_forward_workspace_bytes(int32_t H, int32_t W);
+
+/* cardiax/solve.py:92-100 -- the Euler loop over i = t, t+1, ... while
+ * i < t_end; step i uses stimulus time i. Reads (v_in, w_in, u_in), writes the
+ * final state to (v_out, w_out, u_out). Inputs are not modified; outputs must
+ * not alias inputs or the workspace. `work` holds at least
+ * fk_forward_workspace_bytes(H, W) bytes. */
+int32_t fk_forward_euler(const fk_params* params, const float* D, const float* v_in,
+                         const float* w_in, const float* u_in, int32_t H, int32_t W,
+                         const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                         double t_end, float dt, float dx, float* v_out, float* w_out,
+                         float* u_out, void* work, size_t work_bytes, void* stream);
+
+/* Same loop with the plain one-thread-per-cell kernel (global loads, no LDS,
+ * one launch per step): the unoptimised fp32 baseline the tuned kernels are
+ * measured against. Same contract as fk_forward_euler. */
+int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const float* v_in,
+                               const float* w_in, const float* u_in, int32_t H, int32_t W,
+                               const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                               double t_end, float dt, float dx, float* v_out,
+                               float* w_out, float* u_out, void* work, size_t work_bytes,
+                               void* stream);
+
+/* cardiax/solve.py:225-254 -- first derivative of a (rows, cols) array along
+ * `axis` (0 or 1) without grid spacing; needs >= 5 entries along the axis. */
+int32_t fk_gradient(const float* a, float* out, int32_t rows, int32_t cols, int32_t axis,
+                    void* stream);
+
+/* cardiax/solve.py:257-271 -- out = X with cells replaced by the combined
+ * field of the stimuli active at time t (later stimuli override earlier ones
+ * where their field is nonzero). Fields have n_cells entries. */
+int32_t fk_stimulate(double t, const float* X, float* out, int64_t n_cells,
+                     const fk_stimulus* stimuli, int32_t n_stimuli, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CARDIAX_AMD_H */
