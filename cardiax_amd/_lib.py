@@ -23,6 +23,10 @@ FK_ERR_HIP = -2
 FK_ERR_UNSUPPORTED = -3
 FK_MAX_STIMULI = 32
 
+# Forward kernel variants (fk_set_forward_variant).
+VARIANTS = {"auto": 0, "plain": 1, "tile16": 2, "tile32": 3, "stream8": 4, "stream16": 5,
+            "stream32": 6, "tb64x2": 7, "tb64x4": 8, "tb32x2": 9, "tb32x4": 10}
+
 c_f = ctypes.c_float
 c_d = ctypes.c_double
 c_i32 = ctypes.c_int32
@@ -45,6 +49,7 @@ class FkStimulus(ctypes.Structure):
 _SIGNATURES = {
     "fk_last_error": (ctypes.c_char_p, []),
     "fk_abi_version": (c_i32, []),
+    "fk_set_forward_variant": (c_i32, [c_i32]),
     "fk_init": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "fk_step": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
                         ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_vp, c_vp, c_vp, c_vp]),

@@ -42,22 +42,16 @@ __device__ __forceinline__ float half_one_plus_tanh(const Coef& c, float u) {
 
 // Gates and ionic currents, cardiax/solve.py:35-42,57-58.
 // p = [u >= V_c], q = [u >= V_v]; outputs dv, dw and j_ion (before stimulus).
+// Both sides of every gate are evaluated and selected (no divergent branch).
 __device__ __forceinline__ void reaction(const Coef& c, float v, float w, float u, float& dv,
                                          float& dw, float& jion) {
+  const bool p = u >= c.V_c;
   const float jsi = -(w * half_one_plus_tanh(c, u)) * c.inv_tsi;
-  float jfi, jso;
-  if (u >= c.V_c) {
-    jfi = -v * (u - c.V_c) * (1.0f - u) * c.inv_td;
-    jso = c.inv_tr;
-    dv = -v * c.inv_tvp;
-    dw = -w * c.inv_twp;
-  } else {
-    const float inv_tvm = (u >= c.V_v) ? c.inv_tv2m : c.inv_tv1m;
-    jfi = 0.0f;
-    jso = u * c.inv_t0;
-    dv = (1.0f - v) * inv_tvm;
-    dw = (1.0f - w) * c.inv_twm;
-  }
+  const float jfi = p ? -v * (u - c.V_c) * (1.0f - u) * c.inv_td : 0.0f;
+  const float jso = p ? c.inv_tr : u * c.inv_t0;
+  const float inv_tvm = (u >= c.V_v) ? c.inv_tv2m : c.inv_tv1m;
+  dv = p ? -v * c.inv_tvp : (1.0f - v) * inv_tvm;
+  dw = p ? -w * c.inv_twp : (1.0f - w) * c.inv_twm;
   jion = -(jfi + jso + jsi) * c.inv_cm;
 }
 
@@ -111,6 +105,48 @@ __device__ __forceinline__ float d2_from_d1(G g, int i, int N) {
   if (i == 0) return d1_forward(g(0), g(1), g(2), g(3));
   if (i == N - 1) return d1_backward(g(N - 4), g(N - 3), g(N - 2), g(N - 1));
   return d1_interior(g(i - 2), g(i - 1), g(i + 1), g(i + 2));
+}
+
+// One cell of the step with every rule chosen per position (edge padding,
+// one-sided boundary differences), neighbours read from global memory with
+// clamped indices. Used where a cell is near the boundary and by the plain
+// baseline kernel. EULER: write x + dt * dx/dt, else the derivatives.
+template <bool EULER>
+__device__ __forceinline__ void cell_general(const Coef& c, const float* __restrict__ D,
+                                             const float* __restrict__ v,
+                                             const float* __restrict__ w,
+                                             const float* __restrict__ u, int H, int W, int i,
+                                             int j, const StimArgs& sa, float dt,
+                                             float inv_dx2, float* __restrict__ o_v,
+                                             float* __restrict__ o_w,
+                                             float* __restrict__ o_u) {
+  const size_t idx = (size_t)i * W + j;
+  auto ux = [&](int r) { return u[(size_t)clampi(r, 0, H - 1) * W + j]; };
+  auto uy = [&](int s) { return u[(size_t)i * W + clampi(s, 0, W - 1)]; };
+  auto Dx = [&](int r) { return D[(size_t)clampi(r, 0, H - 1) * W + j]; };
+  auto Dy = [&](int s) { return D[(size_t)i * W + clampi(s, 0, W - 1)]; };
+  auto gx = [&](int q) { return d1_general(ux, q, H); };
+  auto gy = [&](int q) { return d1_general(uy, q, W); };
+  const float uxx = d2_from_d1(gx, i, H);
+  const float uyy = d2_from_d1(gy, j, W);
+  const float dDx = d1_general(Dx, i, H);
+  const float dDy = d1_general(Dy, j, W);
+  const float Dc = D[idx];
+  const float del_u = (Dc * (uxx + uyy) + dDx * gx(i) + dDy * gy(j)) * inv_dx2;
+  const float vc = v[idx], wc = w[idx], uc = u[idx];
+  float dv, dw, jion;
+  reaction(c, vc, wc, uc, dv, dw, jion);
+  jion = stimulate_cell(sa, idx, jion);
+  const float du = del_u + jion;
+  if (EULER) {
+    o_v[idx] = vc + dt * dv;
+    o_w[idx] = wc + dt * dw;
+    o_u[idx] = uc + dt * du;
+  } else {
+    o_v[idx] = dv;
+    o_w[idx] = dw;
+    o_u[idx] = du;
+  }
 }
 
 }  // namespace fk

@@ -12,6 +12,7 @@
 //            HBM pass per step: read v, w, u, D, write v, w, u.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -19,6 +20,8 @@
 #include "cardiax_amd.h"
 #include "fk_device.h"
 #include "fk_host.h"
+#include "fk_stream.h"
+#include "fk_tb.h"
 
 namespace fk {
 
@@ -34,36 +37,7 @@ k_plain(Coef c, const float* __restrict__ D, const float* __restrict__ v,
   const int j = blockIdx.x * 64 + threadIdx.x;
   const int i = blockIdx.y * 4 + threadIdx.y;
   if (i >= H || j >= W) return;
-  const size_t idx = (size_t)i * W + j;
-
-  auto ux = [&](int r) { return u[(size_t)clampi(r, 0, H - 1) * W + j]; };
-  auto uy = [&](int s) { return u[(size_t)i * W + clampi(s, 0, W - 1)]; };
-  auto Dx = [&](int r) { return D[(size_t)clampi(r, 0, H - 1) * W + j]; };
-  auto Dy = [&](int s) { return D[(size_t)i * W + clampi(s, 0, W - 1)]; };
-  auto gx = [&](int q) { return d1_general(ux, q, H); };
-  auto gy = [&](int q) { return d1_general(uy, q, W); };
-
-  const float uxx = d2_from_d1(gx, i, H);
-  const float uyy = d2_from_d1(gy, j, W);
-  const float dDx = d1_general(Dx, i, H);
-  const float dDy = d1_general(Dy, j, W);
-  const float Dc = D[idx];
-  const float del_u = (Dc * (uxx + uyy) + dDx * gx(i) + dDy * gy(j)) * inv_dx2;
-
-  const float vc = v[idx], wc = w[idx], uc = u[idx];
-  float dv, dw, jion;
-  reaction(c, vc, wc, uc, dv, dw, jion);
-  jion = stimulate_cell(sa, idx, jion);
-  const float du = del_u + jion;
-  if (EULER) {
-    o_v[idx] = vc + dt * dv;
-    o_w[idx] = wc + dt * dw;
-    o_u[idx] = uc + dt * du;
-  } else {
-    o_v[idx] = dv;
-    o_w[idx] = dw;
-    o_u[idx] = du;
-  }
+  cell_general<EULER>(c, D, v, w, u, H, W, i, j, sa, dt, inv_dx2, o_v, o_w, o_u);
 }
 
 // ---------------------------------------------------------------------------
@@ -86,8 +60,10 @@ k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
        const float* __restrict__ w, const float* __restrict__ u, int H, int W, StimArgs sa,
        float dt, float inv_dx2, float* __restrict__ o_v, float* __restrict__ o_w,
        float* __restrict__ o_u) {
+  static_assert(TH % 4 == 0, "TH must be a multiple of 4");
   constexpr int UR = TH + 8, UC = TW + 8;  // u, halo 4
   constexpr int DR = TH + 4, DC = TW + 4;  // D, halo 2
+  constexpr int NU = UR / 4, ND = DR / 4, NO = TH / 4;  // rows per wave
   __shared__ float s_u[UR][UC];
   __shared__ float s_D[DR][DC];
   __shared__ float s_gx[TH + 4][TW];  // u_x at tile rows -2 .. TH+1
@@ -98,16 +74,39 @@ k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
   const int r0 = tile_origin(blockIdx.y, TH, H);
   const int c0 = tile_origin(blockIdx.x, TW, W);
 
-  // Stage u and D with clamped indices: this *is* the edge padding.
-  for (int r = wv; r < UR; r += 4) {
-    const float* row = u + (size_t)clampi(r0 - 4 + r, 0, H - 1) * W;
-    s_u[r][lane] = row[clampi(c0 - 4 + lane, 0, W - 1)];
-    if (lane < 8) s_u[r][TW + lane] = row[clampi(c0 + TW - 4 + lane, 0, W - 1)];
+  // Issue every global load of the tile before the first LDS store so their
+  // latencies overlap. Clamped indices *are* the edge padding.
+  float ru[NU], ru2[NU], rD[ND], rD2[ND], rv[NO], rw[NO];
+  const int cu = clampi(c0 - 4 + lane, 0, W - 1), cu2 = clampi(c0 + TW - 4 + lane, 0, W - 1);
+  const int cd = clampi(c0 - 2 + lane, 0, W - 1), cd2 = clampi(c0 + TW - 2 + lane, 0, W - 1);
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const float* row = u + (size_t)clampi(r0 - 4 + wv + 4 * k, 0, H - 1) * W;
+    ru[k] = row[cu];
+    ru2[k] = lane < 8 ? row[cu2] : 0.0f;
   }
-  for (int r = wv; r < DR; r += 4) {
-    const float* row = D + (size_t)clampi(r0 - 2 + r, 0, H - 1) * W;
-    s_D[r][lane] = row[clampi(c0 - 2 + lane, 0, W - 1)];
-    if (lane < 4) s_D[r][TW + lane] = row[clampi(c0 + TW - 2 + lane, 0, W - 1)];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const float* row = D + (size_t)clampi(r0 - 2 + wv + 4 * k, 0, H - 1) * W;
+    rD[k] = row[cd];
+    rD2[k] = lane < 4 ? row[cd2] : 0.0f;
+  }
+  const int gj = c0 + lane, gjc = min(gj, W - 1);
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    const size_t idx = (size_t)min(r0 + wv + 4 * k, H - 1) * W + gjc;
+    rv[k] = v[idx];
+    rw[k] = w[idx];
+  }
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    s_u[wv + 4 * k][lane] = ru[k];
+    if (lane < 8) s_u[wv + 4 * k][TW + lane] = ru2[k];
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    s_D[wv + 4 * k][lane] = rD[k];
+    if (lane < 4) s_D[wv + 4 * k][TW + lane] = rD2[k];
   }
   __syncthreads();
 
@@ -118,12 +117,16 @@ k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
   // u_x on tile rows lr in [-2, TH+2), column `lane`; u_y on rows [0, TH),
   // tile columns lc in [-2, TW+2).
   if (interior) {
-    for (int r = wv; r < TH + 4; r += 4) {
+#pragma unroll
+    for (int k = 0; k < (TH + 4) / 4; ++k) {
+      const int r = wv + 4 * k;
       const int a = r + 2;  // s_u row of tile row lr = r - 2
       s_gx[r][lane] = d1_interior(s_u[a - 2][lane + 4], s_u[a - 1][lane + 4],
                                   s_u[a + 1][lane + 4], s_u[a + 2][lane + 4]);
     }
-    for (int r = wv; r < TH; r += 4) {
+#pragma unroll
+    for (int k = 0; k < TH / 4; ++k) {
+      const int r = wv + 4 * k;
       const float* ur = s_u[r + 4];
       {
         const int b = lane + 2;  // s_u column of tile column lc = lane - 2
@@ -149,8 +152,9 @@ k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
   }
   __syncthreads();
 
-  const int gj = c0 + lane;
-  for (int r = wv; r < TH; r += 4) {
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    const int r = wv + 4 * k;
     const int gi = r0 + r;
     float uxx, uyy, dDx, dDy;
     if (interior) {
@@ -177,7 +181,7 @@ k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
     const float del_u =
         (Dc * (uxx + uyy) + dDx * s_gx[r + 2][lane] + dDy * s_gy[r][lane + 2]) * inv_dx2;
     const size_t idx = (size_t)gi * W + gj;
-    const float vc = v[idx], wc = w[idx], uc = s_u[r + 4][lane + 4];
+    const float vc = rv[k], wc = rw[k], uc = s_u[r + 4][lane + 4];
     float dv, dw, jion;
     reaction(c, vc, wc, uc, dv, dw, jion);
     jion = stimulate_cell(sa, idx, jion);
@@ -194,7 +198,6 @@ k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
   }
 }
 
-constexpr int kTileH = 32;
 
 // ---------------------------------------------------------------------------
 // Small kernels: init, gradient, stimulate
@@ -239,32 +242,111 @@ __global__ void k_stimulate(const float* __restrict__ X, float* __restrict__ out
 // ---------------------------------------------------------------------------
 // Host-side launch helpers
 // ---------------------------------------------------------------------------
-enum class Kernel { Plain, Tile };
+// Kernel variants of one step (the value of fk_set_forward_variant).
+enum Variant : int {
+  kAuto = 0,
+  kPlain = 1,
+  kTile16 = 2,
+  kTile32 = 3,
+  kStream8 = 4,
+  kStream16 = 5,
+  kStream32 = 6,
+  kTb64x2 = 7,   // temporal blocking: 64 x 64 tiles, 2 steps per launch
+  kTb64x4 = 8,   // 64 x 64 tiles, 4 steps per launch
+  kTb32x2 = 9,   // 32 x 32 tiles, 2 steps per launch
+  kTb32x4 = 10,  // 32 x 32 tiles, 4 steps per launch
+  kNumVariants = 11
+};
 
-static hipError_t launch_step(Kernel kind, bool euler, const Coef& c, const float* D,
+static bool is_tb(int v) { return v >= kTb64x2 && v <= kTb32x4; }
+
+static std::atomic<int> g_variant{kAuto};
+
+// Default when no variant is set: the streaming kernel with 8-row bands, the
+// fastest single-step kernel measured on MI355X from 128^2 to 2048^2.
+static int resolve_variant(int v) { return v == kAuto ? (int)kStream8 : v; }
+
+template <int R>
+static void launch_stream(bool euler, const Coef& c, const float* D, const float* v,
+                          const float* w, const float* u, int H, int W, const StimArgs& sa,
+                          float dt, float inv_dx2, float* ov, float* ow, float* ou,
+                          hipStream_t s) {
+  const StreamGeom g = make_stream_geom<R>(H, W);
+  const int blocks = g.blocks;
+  if (euler)
+    k_stream<R, true><<<blocks, 256, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow,
+                                            ou, g);
+  else
+    k_stream<R, false><<<blocks, 256, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow,
+                                             ou, g);
+}
+
+template <int TH>
+static void launch_tile(bool euler, const Coef& c, const float* D, const float* v,
+                        const float* w, const float* u, int H, int W, const StimArgs& sa,
+                        float dt, float inv_dx2, float* ov, float* ow, float* ou,
+                        hipStream_t s) {
+  const dim3 block(64, 4), grid((W + TW - 1) / TW, (H + TH - 1) / TH);
+  if (euler)
+    k_tile<TH, true><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou);
+  else
+    k_tile<TH, false><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow,
+                                             ou);
+}
+
+static hipError_t launch_step(int variant, bool euler, const Coef& c, const float* D,
                               const float* v, const float* w, const float* u, int H, int W,
                               const StimArgs& sa, float dt, float inv_dx2, float* ov,
                               float* ow, float* ou, hipStream_t s) {
-  const dim3 block(64, 4);
-  if (kind == Kernel::Plain) {
-    const dim3 grid((W + 63) / 64, (H + 3) / 4);
-    if (euler)
-      k_plain<true><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou);
-    else
-      k_plain<false><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou);
-  } else {
-    const dim3 grid((W + TW - 1) / TW, (H + kTileH - 1) / kTileH);
-    if (euler)
-      k_tile<kTileH, true><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov,
-                                                  ow, ou);
-    else
-      k_tile<kTileH, false><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov,
-                                                   ow, ou);
+  switch (resolve_variant(variant)) {
+    case kPlain: {
+      const dim3 block(64, 4), grid((W + 63) / 64, (H + 3) / 4);
+      if (euler)
+        k_plain<true><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou);
+      else
+        k_plain<false><<<grid, block, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow,
+                                              ou);
+      break;
+    }
+    case kTile16: launch_tile<16>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
+    case kTile32: launch_tile<32>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
+    case kStream32: launch_stream<32>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
+    case kStream16: launch_stream<16>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
+    case kStream8:
+    default: launch_stream<8>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
   }
   return hipGetLastError();
 }
 
-static int32_t forward_loop(Kernel kind, const fk_params* params, const float* D,
+template <int T, int K, int NT>
+static hipError_t launch_tb(const Coef& c, const float* D, const float* v, const float* w,
+                            const float* u, int H, int W, const TbStim& st, int nsteps,
+                            float dt, float inv_dx2, float* ov, float* ow, float* ou,
+                            hipStream_t s) {
+  const int tiles_x = (W + T - 1) / T, tiles_y = (H + T - 1) / T;
+  const int n = tiles_x * tiles_y;
+  k_tb<T, K, NT><<<n, NT, 0, s>>>(c, D, v, w, u, H, W, st, nsteps, dt, inv_dx2, ov, ow, ou,
+                                  tiles_x, n);
+  return hipGetLastError();
+}
+
+static int tb_depth(int variant) {
+  return (variant == kTb64x4 || variant == kTb32x4) ? 4 : 2;
+}
+
+static hipError_t launch_tb_variant(int variant, const Coef& c, const float* D, const float* v,
+                                    const float* w, const float* u, int H, int W,
+                                    const TbStim& st, int nsteps, float dt, float inv_dx2,
+                                    float* ov, float* ow, float* ou, hipStream_t s) {
+  switch (variant) {
+    case kTb64x2: return launch_tb<64, 2, 1024>(c, D, v, w, u, H, W, st, nsteps, dt, inv_dx2, ov, ow, ou, s);
+    case kTb64x4: return launch_tb<64, 4, 1024>(c, D, v, w, u, H, W, st, nsteps, dt, inv_dx2, ov, ow, ou, s);
+    case kTb32x2: return launch_tb<32, 2, 256>(c, D, v, w, u, H, W, st, nsteps, dt, inv_dx2, ov, ow, ou, s);
+    default: return launch_tb<32, 4, 512>(c, D, v, w, u, H, W, st, nsteps, dt, inv_dx2, ov, ow, ou, s);
+  }
+}
+
+static int32_t forward_loop(int variant, const fk_params* params, const float* D,
                             const float* v_in, const float* w_in, const float* u_in, int32_t H,
                             int32_t W, const fk_stimulus* stimuli, int32_t n_stimuli,
                             double t, double t_end, float dt, float dx, float* v_out,
@@ -292,15 +374,28 @@ static int32_t forward_loop(Kernel kind, const fk_params* params, const float* D
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
   float* wk = (float*)work;
   float* bufs[2][3] = {{v_out, w_out, u_out}, {wk, wk + n, wk + 2 * n}};
-  // Ping-pong so that the last step lands in the output buffers.
-  int dst = (steps % 2 == 1) ? 0 : 1;
+  const int v = resolve_variant(variant);
+  const int depth = is_tb(v) ? tb_depth(v) : 1;
+  const int64_t launches = (steps + depth - 1) / depth;
+  // Ping-pong so that the last launch lands in the output buffers.
+  int dst = (launches % 2 == 1) ? 0 : 1;
   const float* src[3] = {v_in, w_in, u_in};
   StimSchedule sched(stimuli, n_stimuli);
-  for (int64_t k = 0; k < steps; ++k) {
-    const StimArgs& sa = sched.at(t + (double)k);
+  TbStim st{};
+  st.n = n_stimuli;
+  for (int i = 0; i < n_stimuli; ++i) st.f[i] = stimuli[i].field;
+  for (int64_t k = 0; k < steps; k += depth) {
     float** d = bufs[dst];
-    HIP_CHECK(launch_step(kind, true, c, D, src[0], src[1], src[2], H, W, sa, dt, inv_dx2,
-                          d[0], d[1], d[2], s));
+    if (is_tb(v)) {
+      const int ns = (int)std::min<int64_t>(depth, steps - k);
+      for (int j = 0; j < 8; ++j) st.act[j] = j < ns ? sched.mask(t + (double)(k + j)) : 0u;
+      HIP_CHECK(launch_tb_variant(v, c, D, src[0], src[1], src[2], H, W, st, ns, dt, inv_dx2,
+                                  d[0], d[1], d[2], s));
+    } else {
+      const StimArgs& sa = sched.at(t + (double)k);
+      HIP_CHECK(launch_step(v, true, c, D, src[0], src[1], src[2], H, W, sa, dt, inv_dx2,
+                            d[0], d[1], d[2], s));
+    }
     src[0] = d[0];
     src[1] = d[1];
     src[2] = d[2];
@@ -316,6 +411,13 @@ using namespace fk;
 extern "C" {
 
 int32_t fk_abi_version(void) { return 1; }
+
+int32_t fk_set_forward_variant(int32_t variant) {
+  ErrorScope es;
+  if (variant < 0 || variant >= kNumVariants)
+    return set_error(FK_ERR_ARG, "unknown forward variant %d", variant);
+  return g_variant.exchange(variant);
+}
 
 int32_t fk_init(float* v, float* w, float* u, int64_t n_cells, void* stream) {
   ErrorScope es;
@@ -339,7 +441,7 @@ int32_t fk_step(const fk_params* params, const float* D, const float* v, const f
   StimSchedule sched(stimuli, n_stimuli);
   const Coef c = make_coef(*params);
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
-  HIP_CHECK(launch_step(Kernel::Tile, false, c, D, v, w, u, H, W, sched.at(t), 0.0f, inv_dx2,
+  HIP_CHECK(launch_step(g_variant.load(), false, c, D, v, w, u, H, W, sched.at(t), 0.0f, inv_dx2,
                         dv, dw, du, (hipStream_t)stream));
   return FK_OK;
 }
@@ -356,7 +458,7 @@ int32_t fk_step_euler(const fk_params* params, const float* D, const float* v,
   StimSchedule sched(stimuli, n_stimuli);
   const Coef c = make_coef(*params);
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
-  HIP_CHECK(launch_step(Kernel::Tile, true, c, D, v, w, u, H, W, sched.at(t), dt, inv_dx2,
+  HIP_CHECK(launch_step(g_variant.load(), true, c, D, v, w, u, H, W, sched.at(t), dt, inv_dx2,
                         v_out, w_out, u_out, (hipStream_t)stream));
   return FK_OK;
 }
@@ -371,7 +473,7 @@ int32_t fk_forward_euler(const fk_params* params, const float* D, const float* v
                          const fk_stimulus* stimuli, int32_t n_stimuli, double t,
                          double t_end, float dt, float dx, float* v_out, float* w_out,
                          float* u_out, void* work, size_t work_bytes, void* stream) {
-  return forward_loop(Kernel::Tile, params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli, t,
+  return forward_loop(g_variant.load(), params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli, t,
                       t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
 }
 
@@ -381,7 +483,7 @@ int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const fl
                                double t_end, float dt, float dx, float* v_out,
                                float* w_out, float* u_out, void* work, size_t work_bytes,
                                void* stream) {
-  return forward_loop(Kernel::Plain, params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli,
+  return forward_loop(kPlain, params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli,
                       t, t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
 }
 

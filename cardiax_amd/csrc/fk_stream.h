@@ -1,0 +1,223 @@
+// Streaming forward-step kernel: one wavefront per (column strip, row band).
+//
+// Layout. Lane l of a wave holds column cs - 4 + l: 56 output columns plus 4
+// halo columns on each side. The wave walks the R rows of its band.
+//   * x (rows, axis 0): every lane keeps windows of u (rows rs-4 .. rs+R+3),
+//     u_x (rs-2 .. rs+R+1) and D (rs-2 .. rs+R+1) in registers;
+//     u_x(q) = G(u)(q), u_xx(i) = G(u_x)(i).
+//   * y (columns, axis 1): neighbours +-1..3 columns away come from the lanes
+//     next door through DPP wavefront shifts (v_mov_b32 wave_shr / wave_shl),
+//     so u_y, u_yy and D_y need neither LDS nor a barrier.
+// Row indices are wave-uniform (readfirstlane), so every load and store is a
+// scalar row base plus the lane's column offset. All loads of a band are
+// issued before its arithmetic.
+//
+// Boundary rules (cardiax/solve.py:225-254: one-sided differences on the two
+// outermost padded entries, edge padding by one cell) only matter within 3
+// cells of an edge. Bands and strips are placed so that the first starts at 0
+// and the last ends at N (shifted back, overlapping its neighbour with
+// identical results), and a band / strip that touches an edge runs the
+// general per-position rules through a wave-uniform template switch; all
+// others run the central rules only.
+#pragma once
+
+#include "fk_device.h"
+
+namespace fk {
+
+constexpr int kStripOut = 56;  // output columns per wave
+
+// Lane l receives the value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1).
+// The lane without a source keeps 0; it is a halo lane whose result is unused.
+__device__ __forceinline__ float from_left(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+
+// First difference across columns of a value held one per lane. GENERAL:
+// the lane's column `col` selects the rule, as d1_general does.
+template <bool GENERAL>
+__device__ __forceinline__ float d1_lanes(float x, int col, int W) {
+  const float m1 = from_left(x), p1 = from_right(x);
+  const float m2 = from_left(m1), p2 = from_right(p1);
+  const float ctr = d1_interior(m2, m1, p1, p2);
+  if (!GENERAL) return ctr;
+  const float m3 = from_left(m2), p3 = from_right(p2);
+  const float fwd = d1_forward(x, p1, p2, p3);
+  const float bwd = d1_backward(m3, m2, m1, x);
+  return col <= 0 ? fwd : (col >= W - 1 ? bwd : ctr);
+}
+
+// Second difference across columns from first differences g held one per
+// lane (cardiax/solve.py:51-52 applied to u_y): one-sided at columns 0, W-1.
+template <bool GENERAL>
+__device__ __forceinline__ float d2_lanes(float g, int col, int W) {
+  const float m1 = from_left(g), p1 = from_right(g);
+  const float m2 = from_left(m1), p2 = from_right(p1);
+  const float ctr = d1_interior(m2, m1, p1, p2);
+  if (!GENERAL) return ctr;
+  const float m3 = from_left(m2), p3 = from_right(p2);
+  const float fwd = d1_forward(g, p1, p2, p3);
+  const float bwd = d1_backward(m3, m2, m1, g);
+  return col == 0 ? fwd : (col == W - 1 ? bwd : ctr);
+}
+
+// Work decomposition of one launch, computed on the host.
+struct StreamGeom {
+  int n_strips, n_bands, n_tiles, blocks;
+  int cs_last, rs_last;  // origin of the last strip / band (shifted back to end at N)
+};
+
+template <int R>
+inline StreamGeom make_stream_geom(int H, int W) {
+  StreamGeom g{};
+  g.n_strips = (W + kStripOut - 1) / kStripOut;
+  g.n_bands = (H + R - 1) / R;
+  g.cs_last = W >= kStripOut ? W - kStripOut : 0;
+  g.rs_last = H >= R ? H - R : 0;
+  g.n_tiles = g.n_strips * g.n_bands;
+  g.blocks = (g.n_tiles + 3) / 4;
+  return g;
+}
+
+// Load one float at byte offset `off` (32-bit, unsigned) from a wave-uniform
+// row base: this is the scalar-base + vector-offset addressing form.
+__device__ __forceinline__ float ld(const float* rowp, uint32_t off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(rowp) + off);
+}
+__device__ __forceinline__ void st(float* rowp, uint32_t off, float x) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(rowp) + off) = x;
+}
+
+template <int R, bool EULER, bool XG, bool YG, bool STIM>
+__device__ __forceinline__ void stream_tile(const Coef& c, const float* __restrict__ D,
+                                            const float* __restrict__ v,
+                                            const float* __restrict__ w,
+                                            const float* __restrict__ u, int H, int W,
+                                            const StimArgs& sa, float dt, float inv_dx2,
+                                            float* __restrict__ o_v, float* __restrict__ o_w,
+                                            float* __restrict__ o_u, int rs, int cs) {
+  const int lane = threadIdx.x & 63;
+  const int col = cs - 4 + lane;                           // this lane's column
+  const uint32_t off = (uint32_t)clampi(col, 0, W - 1) * 4u;  // clamped = edge padding
+  const bool out_col = lane >= 4 && lane < 4 + kStripOut && col < W;
+  auto row = [&](const float* a, int r) {  // wave-uniform row base, clamped
+    return a + (size_t)clampi(r, 0, H - 1) * (size_t)W;
+  };
+
+  float uu[R + 8];  // u rows rs-4 .. rs+R+3
+  float dd[R + 4];  // D rows rs-2 .. rs+R+1
+  float vv[R], ww[R];
+#pragma unroll
+  for (int k = 0; k < R + 8; ++k) uu[k] = ld(row(u, rs - 4 + k), off);
+#pragma unroll
+  for (int k = 0; k < R + 4; ++k) dd[k] = ld(row(D, rs - 2 + k), off);
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    vv[k] = ld(row(v, rs + k), off);
+    ww[k] = ld(row(w, rs + k), off);
+  }
+
+  // u_x at rows q = rs-2 .. rs+R+1
+  float gx[R + 4];
+#pragma unroll
+  for (int k = 0; k < R + 4; ++k) {
+    const int q = rs - 2 + k;
+    if (XG && k <= 2 && q <= 0)  // forward rule reads u(q..q+3); q <= 0 forces k <= 2
+      gx[k] = d1_forward(uu[k + 2], uu[k + 3], uu[k + 4], uu[k + 5]);
+    else if (XG && k >= 3 && q >= H - 1)  // backward rule reads u(q-3..q)
+      gx[k] = d1_backward(uu[k - 1], uu[k], uu[k + 1], uu[k + 2]);
+    else
+      gx[k] = d1_interior(uu[k], uu[k + 1], uu[k + 3], uu[k + 4]);
+  }
+
+  // All rows first (one straight-line block for the scheduler), stores after.
+  float rv[R], rw[R], ru[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int i = rs + k;
+    const float uc = uu[k + 4], Dc = dd[k + 2], gxc = gx[k + 2];
+    float uxx, dDx;
+    if (XG && k == 0 && i == 0) {  // i = rs + k = 0 forces k = 0
+      uxx = d1_forward(gx[k + 2], gx[k + 3], gx[k + 4], gx[k + 5]);
+      dDx = d1_forward(dd[k + 2], dd[k + 3], dd[k + 4], dd[k + 5]);
+    } else if (XG && k >= 1 && i == H - 1) {
+      uxx = d1_backward(gx[k - 1], gx[k], gx[k + 1], gx[k + 2]);
+      dDx = d1_backward(dd[k - 1], dd[k], dd[k + 1], dd[k + 2]);
+    } else {
+      uxx = d1_interior(gx[k], gx[k + 1], gx[k + 3], gx[k + 4]);
+      dDx = d1_interior(dd[k], dd[k + 1], dd[k + 3], dd[k + 4]);
+    }
+    const float gyc = d1_lanes<YG>(uc, col, W);
+    const float uyy = d2_lanes<YG>(gyc, col, W);
+    const float dDy = d1_lanes<YG>(Dc, col, W);
+    const float del_u = (Dc * (uxx + uyy) + dDx * gxc + dDy * gyc) * inv_dx2;
+    float dv, dw, jion;
+    reaction(c, vv[k], ww[k], uc, dv, dw, jion);
+    if (STIM) {
+      float sv = 0.0f;
+      for (int m = 0; m < sa.n; ++m) {
+        const float f = ld(row(sa.f[m], i), off);
+        sv = (f != 0.0f) ? f : sv;
+      }
+      jion = (sv != 0.0f) ? sv : jion;
+    }
+    const float du = del_u + jion;
+    if (EULER) {
+      rv[k] = fmaf(dt, dv, vv[k]);
+      rw[k] = fmaf(dt, dw, ww[k]);
+      ru[k] = fmaf(dt, du, uc);
+    } else {
+      rv[k] = dv;
+      rw[k] = dw;
+      ru[k] = du;
+    }
+  }
+  if (out_col) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int i = rs + k;
+      if (XG && i >= H) break;  // rows past the grid (only when H < R)
+      const size_t ro = (size_t)i * (size_t)W;
+      st(o_v + ro, off, rv[k]);
+      st(o_w + ro, off, rw[k]);
+      st(o_u + ro, off, ru[k]);
+    }
+  }
+}
+
+// Four waves per block, one (strip, band) tile each, band-major within a
+// strip so that consecutive tiles share row halos.
+template <int R, bool EULER>
+__global__ void __launch_bounds__(256)
+k_stream(Coef c, const float* __restrict__ D, const float* __restrict__ v,
+         const float* __restrict__ w, const float* __restrict__ u, int H, int W, StimArgs sa,
+         float dt, float inv_dx2, float* __restrict__ o_v, float* __restrict__ o_w,
+         float* __restrict__ o_u, StreamGeom g) {
+  const int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x >> 6));
+  if (tile >= g.n_tiles) return;  // whole wave leaves; the kernel has no barrier
+  const int strip = tile / g.n_bands, band = tile - strip * g.n_bands;
+  const int cs = min(kStripOut * strip, g.cs_last);
+  const int rs = min(R * band, g.rs_last);
+  const bool xg = rs < 3 || rs + R + 3 > H;          // band within 3 rows of an edge
+  const bool yg = cs < 3 || cs + kStripOut + 3 > W;  // strip within 3 columns
+  const int sel = (xg ? 1 : 0) | (yg ? 2 : 0) | (sa.n ? 4 : 0);
+#define FK_TILE(X, Y, S)                                                                   \
+  stream_tile<R, EULER, X, Y, S>(c, D, v, w, u, H, W, sa, dt, inv_dx2, o_v, o_w, o_u, rs, \
+                                 cs)
+  switch (sel) {
+    case 0: FK_TILE(false, false, false); break;
+    case 1: FK_TILE(true, false, false); break;
+    case 2: FK_TILE(false, true, false); break;
+    case 3: FK_TILE(true, true, false); break;
+    case 4: FK_TILE(false, false, true); break;
+    case 5: FK_TILE(true, false, true); break;
+    case 6: FK_TILE(false, true, true); break;
+    default: FK_TILE(true, true, true); break;
+  }
+#undef FK_TILE
+}
+
+}  // namespace fk

@@ -62,6 +62,16 @@ const char* fk_last_error(void);
 /* ABI revision of this library (incremented on incompatible change). */
 int32_t fk_abi_version(void);
 
+/* Kernel used for the forward step by fk_step, fk_step_euler and
+ * fk_forward_euler: 0 = automatic (default), 1 = plain one thread per cell,
+ * 2 / 3 = LDS tile of 16 / 32 rows, 4 / 5 / 6 = streaming wavefront bands of
+ * 8 / 16 / 32 rows, 7 / 8 / 9 / 10 = temporal blocking on 64x64 tiles (2 / 4
+ * steps per launch) and 32x32 tiles (2 / 4 steps per launch; fk_step and
+ * fk_step_euler use variant 4 then). Process-wide; set by the autotuner. No reference
+ * counterpart (the reference leaves kernel choice to XLA). Returns the
+ * previous setting, or FK_ERR_ARG for an unknown variant. */
+int32_t fk_set_forward_variant(int32_t variant);
+
 /* cardiax/solve.py:18-23 -- v = w = 1, u = 0 over n_cells cells. */
 int32_t fk_init(float* v, float* w, float* u, int64_t n_cells, void* stream);
 

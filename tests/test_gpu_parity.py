@@ -17,6 +17,20 @@ from oracle import c_oracle, fk_oracle as o
 
 pytestmark = pytest.mark.gpu
 
+KERNEL_VARIANTS = ["plain", "tile16", "tile32", "stream8", "stream16", "stream32",
+                   "tb64x2", "tb64x4", "tb32x2", "tb32x4"]
+
+
+@pytest.fixture(params=KERNEL_VARIANTS)
+def variant(request, gpu):
+    """Run the test with one forward kernel variant, then restore the default."""
+    from cardiax_amd import _lib
+
+    L = _lib.lib()
+    prev = L.fk_set_forward_variant(_lib.VARIANTS[request.param])
+    yield request.param
+    L.fk_set_forward_variant(prev)
+
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
 
 
@@ -70,12 +84,15 @@ def test_stimulate(gpu):
         np.testing.assert_array_equal(got, o.stimulate(t, X.astype(np.float64), ostims))
 
 
-SHAPES = [(3, 3), (5, 7), (16, 16), (33, 65), (64, 64), (100, 37), (130, 200), (37, 129)]
+# includes stages that end exactly on the grid edge: H = (ty+1) T + 4K for
+# 64-tiles with K = 2 (72, 136) and K = 4 (80), 32-tiles with K = 2 / 4 (40, 48)
+SHAPES = [(3, 3), (5, 7), (16, 16), (33, 65), (64, 64), (100, 37), (130, 200), (37, 129),
+          (72, 80), (136, 48), (40, 200)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("pname", ["3", "1A", "7"])
-def test_step_derivatives(gpu, shape, pname):
+def test_step_derivatives(variant, shape, pname):
     prm = P.ALL_PARAMSETS[pname]
     st = f32(rand_state(shape, 11))
     rng = np.random.default_rng(12)
@@ -92,7 +109,7 @@ def test_step_derivatives(gpu, shape, pname):
 
 
 @pytest.mark.parametrize("shape", [(3, 4), (45, 70), (129, 64)])
-def test_step_euler(gpu, shape):
+def test_step_euler(variant, shape):
     st = f32(rand_state(shape, 3))
     D = np.full(shape, 1e-3, np.float32)
     got = solve.step_euler(st, 5, P.PARAMSET_3, D, [], 0.01, 0.01)
@@ -104,7 +121,7 @@ def test_step_euler(gpu, shape):
 
 @pytest.mark.parametrize("name", ["fk_16_p3_west", "fk_32_p1a_overlap", "fk_48x40_p5_edges",
                                   "fk_64_p7_s1s2", "fk_5x7_p3_tiny"])
-def test_golden_fixtures(gpu, name):
+def test_golden_fixtures(variant, name):
     z = np.load("%s/%s.npz" % (GOLDEN, name))
     prm = P.Params(*z["params"])
     stims = [(Protocol(*p), f) for p, f in zip(z["protocols"], z["fields"])]
@@ -118,6 +135,27 @@ def test_golden_fixtures(gpu, name):
             err = np.abs(npy(g) - r).max()
             assert err <= 1e-4 * scale, (name, int(c), "vwu"[k], err)
         t = int(c)
+
+
+@pytest.mark.parametrize("shape", [(72, 80), (136, 48), (40, 200), (97, 131), (5, 64)])
+def test_forward_matches_oracle_odd_shapes(variant, shape):
+    """60 Euler steps from a perturbed state with two stimuli on grids whose
+    tiles / stages end on or near the edges."""
+    H, W = shape
+    ii, jj = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    u0 = 0.5 + 0.45 * np.sin(2 * np.pi * ii / max(H, 8) * 1.3) * np.cos(2 * np.pi * jj / W * 0.7)
+    st = f32(o.State(0.8 + 0.1 * np.cos(jj / 7.0), 0.9 - 0.05 * np.sin(ii / 5.0), u0))
+    rng = np.random.default_rng(22)
+    D = rng.uniform(6e-4, 1e-3, shape).astype(np.float32)
+    f1 = np.zeros(shape, np.float32); f1[:, :3] = 20.0
+    f2 = np.zeros(shape, np.float32); f2[-4:, :] = -2.0
+    stims = [(Protocol(0, 2, 1e9), f1), (Protocol(5, 3, 20), f2)]
+    got = solve._forward_euler(st, 0, 60, P.PARAMSET_3, D, stims, 0.01, 0.01)
+    ref = o.forward_euler(o.State(*[x.astype(np.float64) for x in st]), 0, 60, P.PARAMSET_3,
+                          D.astype(np.float64),
+                          [o.Stimulus(o.Protocol(*p), f) for p, f in stims], 0.01, 0.01)
+    for name, g, r in zip("vwu", got, ref):
+        assert np.abs(npy(g) - r).max() <= 1e-4 * max(1.0, np.abs(r).max()), name
 
 
 def test_plain_and_tile_kernels_agree(gpu):
@@ -141,7 +179,7 @@ def _activation(us, theta=0.5):
     return act
 
 
-def test_c1_parity_1000_steps(gpu):
+def test_c1_parity_1000_steps(variant):
     """128^2, PARAMSET_3, S1 + S2, 1000 steps: u within 1e-4 of the FP64
     oracle every 100 steps; activation times within 1 step."""
     sim = wl.c1_inputs(0)
@@ -244,11 +282,13 @@ def test_argument_errors(gpu):
     with pytest.raises(NotImplementedError):
         solve._forward_euler(st, 0, 5, P.PARAMSET_3, np.ones((10, 12), np.float32), many,
                              0.01, 0.01)
-    with pytest.raises(ValueError):
-        solve.init((2, 5))
+    small = solve.init((2, 5))  # like jnp.ones, init accepts any shape ...
+    with pytest.raises(ValueError, match="too small"):  # ... the solver needs >= 3 x 3
+        solve._forward_euler(small, 0, 1, P.PARAMSET_3, np.ones((2, 5), np.float32), [],
+                             0.01, 0.01)
 
 
-def test_full_size_mirror_symmetry_1024(gpu):
+def test_full_size_mirror_symmetry_1024(variant):
     """1024^2 scarred C2 inputs, 500 steps: solving the left-right mirrored
     problem gives the mirrored solution (size-independent property)."""
     sim = wl.c2_inputs(0, steps=500)
@@ -266,13 +306,19 @@ def test_full_size_mirror_symmetry_1024(gpu):
 
 def test_full_size_planar_wave_row_invariance_1024(gpu):
     """Uniform D with a WEST stripe: every row of the 1024^2 solution is the
-    same 1-D solution, up to the rounding of the edge rules."""
+    same 1-D solution. The one-sided rules applied to a constant column round
+    to ~1e-7 instead of 0, so the 3-4 edge rows drift first and the excitable
+    front carries that inwards over time; until it arrives the bulk rows are
+    bitwise identical, which rules out any row-dependent arithmetic."""
     shape = (1024, 1024)
     f = np.zeros(shape, np.float32); f[:, :50] = 20.0
-    st = solve.init(shape)
-    out = solve._forward_euler(st, 0, 1500, P.PARAMSET_3, np.full(shape, 1e-3, np.float32),
+    D = np.full(shape, 1e-3, np.float32)
+    early = solve._forward_euler(solve.init(shape), 0, 10, P.PARAMSET_3, D,
+                                 [(Protocol(0, 2, 1e9), f)], 0.01, 0.01)
+    for x in early:
+        assert torch.equal(x[8:1016], x[512:513].expand(1008, -1))
+    out = solve._forward_euler(early, 10, 1500, P.PARAMSET_3, D,
                                [(Protocol(0, 2, 1e9), f)], 0.01, 0.01)
     for x in out:
-        dev = (x - x[512:513]).abs().max()
-        assert float(dev) <= 1e-5
+        assert float((x - x[512:513]).abs().max()) <= 1e-4
     assert float(out.u.max()) > 0.5
