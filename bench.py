@@ -64,6 +64,53 @@ def cpu_baseline(sim, target_s=12.0):
                       % (H, W, n0, n0 + n1, cores, el)}
 
 
+def bench_batched(n_sims, n_steps, seed, per_sim_steps=64):
+    """256 independent simulations, sides from {64,128,256,384,512}, seeded
+    parameter sets, scar maps and S1/S2 timings: one batched graph-replayed
+    run of n_steps against per-simulation launches (one _forward_euler call
+    per simulation) over per_sim_steps steps."""
+    import torch
+
+    from cardiax_amd import solve, workloads
+    from cardiax_amd.batch import Batch, BatchSim
+
+    dev = torch.device("cuda", 0)
+    sims = workloads.c3_inputs(seed, n=n_sims, steps=n_steps)
+    bs = []
+    for s in sims:
+        stims = [(x.protocol, torch.from_numpy(x.field).to(dev)) for x in s.stimuli]
+        bs.append(BatchSim(solve.init(s.shape), s.params, torch.from_numpy(s.D).to(dev),
+                           stims, float(s.t0), s.dt, s.dx))
+    cells = sum(s.D.size for s in sims)
+    with Batch(bs) as b:
+        b.run(64)  # warm-up: records the graph
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run(n_steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        finite = all(bool(torch.isfinite(x.u).all().item()) for x in b.states)
+    batched = cells * n_steps / el / 1e9
+    # per-simulation launches: the same simulations, one call each
+    for x in bs[:4]:
+        solve._forward_euler(x.state, 0, 4, x.params, x.diffusivity, x.stimuli, x.dt, x.dx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for x in bs:
+        solve._forward_euler(x.state, x.t, x.t + per_sim_steps, x.params, x.diffusivity,
+                             x.stimuli, x.dt, x.dx)
+    torch.cuda.synchronize()
+    el1 = time.perf_counter() - t0
+    per_sim = cells * per_sim_steps / el1 / 1e9
+    return {"workload": "%d simulations, sides %s, %.1fM cells, %d Euler steps"
+                        % (n_sims, list(workloads.C3_SIDES), cells / 1e6, n_steps),
+            "gcups": round(batched, 3), "seconds": round(el, 3),
+            "per_sim_launch_gcups": round(per_sim, 3),
+            "per_sim_launch_steps": per_sim_steps,
+            "speedup_vs_per_sim": round(batched / per_sim, 2), "u_finite": finite,
+            "hbm_frac": round(batched * BYTES_PER_CELL_UPDATE / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +122,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", action="store_true",
                     help="also time the plain one-thread-per-cell kernel")
+    ap.add_argument("--batch-sims", type=int, default=256,
+                    help="simulations of the batched workload (0 = skip it)")
+    ap.add_argument("--batch-steps", type=int, default=5000)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -142,6 +192,11 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         extra["plain_kernel_gcups"] = round(H * W * n / (e0.elapsed_time(e1) / 1e3) / 1e9, 3)
+
+    if args.batch_sims > 0:
+        log("batched workload ...")
+        extra["batched"] = bench_batched(args.batch_sims, args.batch_steps, args.seed)
+        log("batched: %s" % json.dumps(extra["batched"]))
 
     cpu = None
     if not args.no_cpu:

@@ -45,6 +45,17 @@ class FkStimulus(ctypes.Structure):
     _fields_ = [("start", c_d), ("duration", c_d), ("period", c_d), ("field", c_vp)]
 
 
+FK_BATCH_MAX_STIMULI = 4
+
+
+class FkSim(ctypes.Structure):
+    """struct fk_sim: one member of a batch."""
+    _fields_ = [("params", FkParams), ("D", c_vp), ("v", c_vp), ("w", c_vp), ("u", c_vp),
+                ("H", c_i32), ("W", c_i32), ("n_stimuli", c_i32),
+                ("stimuli", FkStimulus * FK_BATCH_MAX_STIMULI), ("t", c_d), ("dt", c_f),
+                ("dx", c_f)]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "fk_last_error": (ctypes.c_char_p, []),
@@ -65,6 +76,10 @@ _SIGNATURES = {
                                        c_d, c_f, c_f, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
     "fk_gradient": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp]),
     "fk_stimulate": (c_i32, [c_d, c_vp, c_vp, c_i64, ctypes.POINTER(FkStimulus), c_i32, c_vp]),
+    "fk_batch_create": (c_i32, [ctypes.POINTER(FkSim), c_i32, ctypes.POINTER(c_vp)]),
+    "fk_batch_run": (c_i32, [c_vp, c_i64, c_i32, c_vp]),
+    "fk_batch_n_items": (c_i32, [c_vp]),
+    "fk_batch_destroy": (c_i32, [c_vp]),
 }
 
 _lib = None

@@ -83,20 +83,34 @@ inline StreamGeom make_stream_geom(int H, int W) {
 }
 
 // Load one float at byte offset `off` (32-bit, unsigned) from a wave-uniform
-// row base: this is the scalar-base + vector-offset addressing form.
+// row base: the scalar-base + vector-offset addressing form. Row pointers are
+// taken as global-memory pointers (every field lives in device memory), so
+// pointers read from a descriptor also compile to global, not flat, access.
+typedef const char __attribute__((address_space(1)))* gcbytes;
+typedef char __attribute__((address_space(1)))* gbytes;
 __device__ __forceinline__ float ld(const float* rowp, uint32_t off) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(rowp) + off);
+  return *(const float __attribute__((address_space(1)))*)((gcbytes)rowp + off);
 }
 __device__ __forceinline__ void st(float* rowp, uint32_t off, float x) {
-  *reinterpret_cast<float*>(reinterpret_cast<char*>(rowp) + off) = x;
+  *(float __attribute__((address_space(1)))*)((gbytes)rowp + off) = x;
 }
 
-template <int R, bool EULER, bool XG, bool YG, bool STIM>
+// Stimulus fields seen by a tile: count(), on(m) (field m active this step)
+// and field(m), in list order. This view reads the active list passed as a
+// kernel argument.
+struct StimListView {
+  const StimArgs& sa;
+  __device__ int count() const { return sa.n; }
+  __device__ bool on(int) const { return true; }
+  __device__ const float* field(int m) const { return sa.f[m]; }
+};
+
+template <int R, bool EULER, bool XG, bool YG, bool STIM, class SV>
 __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restrict__ D,
                                             const float* __restrict__ v,
                                             const float* __restrict__ w,
                                             const float* __restrict__ u, int H, int W,
-                                            const StimArgs& sa, float dt, float inv_dx2,
+                                            const SV& sv, float dt, float inv_dx2,
                                             float* __restrict__ o_v, float* __restrict__ o_w,
                                             float* __restrict__ o_u, int rs, int cs) {
   const int lane = threadIdx.x & 63;
@@ -133,8 +147,6 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
       gx[k] = d1_interior(uu[k], uu[k + 1], uu[k + 3], uu[k + 4]);
   }
 
-  // All rows first (one straight-line block for the scheduler), stores after.
-  float rv[R], rw[R], ru[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int i = rs + k;
@@ -157,33 +169,26 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
     float dv, dw, jion;
     reaction(c, vv[k], ww[k], uc, dv, dw, jion);
     if (STIM) {
-      float sv = 0.0f;
-      for (int m = 0; m < sa.n; ++m) {
-        const float f = ld(row(sa.f[m], i), off);
-        sv = (f != 0.0f) ? f : sv;
+      float sval = 0.0f;
+      for (int m = 0; m < sv.count(); ++m) {
+        if (!sv.on(m)) continue;
+        const float f = ld(row(sv.field(m), i), off);
+        sval = (f != 0.0f) ? f : sval;
       }
-      jion = (sv != 0.0f) ? sv : jion;
+      jion = (sval != 0.0f) ? sval : jion;
     }
     const float du = del_u + jion;
-    if (EULER) {
-      rv[k] = fmaf(dt, dv, vv[k]);
-      rw[k] = fmaf(dt, dw, ww[k]);
-      ru[k] = fmaf(dt, du, uc);
-    } else {
-      rv[k] = dv;
-      rw[k] = dw;
-      ru[k] = du;
-    }
-  }
-  if (out_col) {
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int i = rs + k;
-      if (XG && i >= H) break;  // rows past the grid (only when H < R)
+    if (out_col && (!XG || i < H)) {  // rows past the grid only when H < R
       const size_t ro = (size_t)i * (size_t)W;
-      st(o_v + ro, off, rv[k]);
-      st(o_w + ro, off, rw[k]);
-      st(o_u + ro, off, ru[k]);
+      if (EULER) {
+        st(o_v + ro, off, fmaf(dt, dv, vv[k]));
+        st(o_w + ro, off, fmaf(dt, dw, ww[k]));
+        st(o_u + ro, off, fmaf(dt, du, uc));
+      } else {
+        st(o_v + ro, off, dv);
+        st(o_w + ro, off, dw);
+        st(o_u + ro, off, du);
+      }
     }
   }
 }
@@ -204,8 +209,9 @@ k_stream(Coef c, const float* __restrict__ D, const float* __restrict__ v,
   const bool xg = rs < 3 || rs + R + 3 > H;          // band within 3 rows of an edge
   const bool yg = cs < 3 || cs + kStripOut + 3 > W;  // strip within 3 columns
   const int sel = (xg ? 1 : 0) | (yg ? 2 : 0) | (sa.n ? 4 : 0);
+  const StimListView sv{sa};
 #define FK_TILE(X, Y, S)                                                                   \
-  stream_tile<R, EULER, X, Y, S>(c, D, v, w, u, H, W, sa, dt, inv_dx2, o_v, o_w, o_u, rs, \
+  stream_tile<R, EULER, X, Y, S>(c, D, v, w, u, H, W, sv, dt, inv_dx2, o_v, o_w, o_u, rs, \
                                  cs)
   switch (sel) {
     case 0: FK_TILE(false, false, false); break;

@@ -60,18 +60,21 @@ def rect_mask(shape, centre, size, modulus):
     return m
 
 
-def scar_map(shape, rng, n_min=1, n_max=5, axes=(20, 200), scale=(0.02, 0.2), base=D0,
-             taper=(3, 5), taper_frac=0.03):
+def scar_map(shape, rng, n_min=1, n_max=5, axes=(20, 200), scale=(0.1, 0.2), base=D0,
+             taper=(3, 5), taper_frac=0.1):
     """Baseline diffusivity with elliptical scars.
 
     Each scar: uniform centre, full axes U[axes] cells (scaled by side/1024 for
     grids other than 1024), rotation U[0, pi), D multiplied inside by
     U[scale], with a smoothstep edge max(U[taper], taper_frac * side) cells
-    wide. The edge widens with the grid like the reference's side-proportional
-    Gaussian blur (deepx/utils_scars.py:402-426), and the scale stays above
-    0.02: the reference's non-conservative D (u_xx + u_yy) + D_x u_x + D_y u_y
-    with its stencils has growing modes where D drops steeply towards 0, which
-    excited tissue does not damp.
+    wide. The edge scales with the grid like the reference's blur, whose
+    kernel spans 10% of the image (deepx/utils_scars.py:402-426), and the
+    scale stays >= 0.1, the floor of the reference's own random D range
+    [1e-4, 1e-3] (deepx/generate.py:80). Both keep the dynamics bounded: the
+    reference's non-conservative D (u_xx + u_yy) + D_x u_x + D_y u_y with its
+    stencils has growing modes where D changes steeply (edges of 3% of the
+    side, or scales near 0, diverge on 64^2 grids for most parameter sets),
+    which excited tissue does not damp.
     """
     H, W = shape
     s = min(H, W) / 1024.0

@@ -125,6 +125,51 @@ int32_t fk_gradient(const float* a, float* out, int32_t rows, int32_t cols, int3
 int32_t fk_stimulate(double t, const float* X, float* out, int64_t n_cells,
                      const fk_stimulus* stimuli, int32_t n_stimuli, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched forward Euler. No reference counterpart: the reference runs one
+ * simulation per call (experiments/generate_fd_data_128.py:25-56 loops over
+ * seeds). Each simulation follows cardiax/solve.py:92-100 exactly as
+ * fk_forward_euler does.
+ * ------------------------------------------------------------------------- */
+
+/* Most stimuli one simulation of a batch may carry. */
+#define FK_BATCH_MAX_STIMULI 4
+
+/* One simulation of a batch: its own constants, D-map, state (updated in
+ * place by fk_batch_run), stimuli, time of the next step and steps. */
+typedef struct fk_sim {
+  fk_params params;
+  const float* D;
+  float* v;
+  float* w;
+  float* u;
+  int32_t H, W;
+  int32_t n_stimuli;
+  fk_stimulus stimuli[FK_BATCH_MAX_STIMULI];
+  double t;
+  float dt, dx;
+} fk_sim;
+
+typedef struct fk_batch fk_batch; /* opaque */
+
+/* Builds the tile work list of all simulations and allocates the device
+ * descriptors and ping-pong state pools (the only allocating entry point;
+ * fk_batch_destroy frees them). The fields and state buffers the descriptors
+ * point to must outlive the batch. */
+int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out);
+
+/* Advances every simulation by n_steps Euler steps; results land in each
+ * fk_sim's v, w, u and each simulation's time advances by n_steps.
+ * use_graph != 0 replays a recorded hipGraph of 32 steps (one graph launch
+ * per 32 steps for the whole batch), otherwise one launch per step. */
+int32_t fk_batch_run(fk_batch* b, int64_t n_steps, int32_t use_graph, void* stream);
+
+/* Number of (simulation, strip, band) tiles of one batch step. */
+int32_t fk_batch_n_items(const fk_batch* b);
+
+/* Waits for the device and releases the batch. */
+int32_t fk_batch_destroy(fk_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
