@@ -111,6 +111,61 @@ def bench_batched(n_sims, n_steps, seed, per_sim_steps=64):
             "hbm_frac": round(batched * BYTES_PER_CELL_UPDATE / HBM_PEAK_GBS, 4)}
 
 
+def bench_adjoint(seed, size=256, n_steps=2000, frame_every=50, reps=2):
+    """Gradient of the frame MSE with respect to D (size^2) and tau_d, tau_r,
+    tau_si by the hand-written adjoint with sqrt(N) checkpoints, against the
+    plain forward of the same rollout."""
+    import numpy as np
+    import torch
+
+    from cardiax_amd import adjoint, solve, workloads
+    from cardiax_amd.params import PARAMSET_3 as prm
+
+    dev = torch.device("cuda", 0)
+    shape = (size, size)
+    rng = np.random.default_rng(seed)
+    Dstar = torch.from_numpy(workloads.scar_map(shape, rng)).to(dev)
+    sim = workloads.c1_inputs(seed, shape=shape, steps=n_steps)
+    stims = [(x.protocol, torch.from_numpy(x.field).to(dev)) for x in sim.stimuli]
+    st = solve.init(shape)
+    cps = list(range(0, n_steps + 1, frame_every))
+    target = torch.stack([x.u for x in solve.forward(st, cps, prm, Dstar, stims, 0.01, 0.01)])
+    D = torch.full(shape, 1e-3, device=dev, requires_grad=True)
+    taus = [torch.tensor(float(getattr(prm, k)), device=dev, requires_grad=True)
+            for k in adjoint.GRAD_FIELDS]
+
+    def fwd_only():
+        with torch.no_grad():
+            return adjoint.rollout(D, *taus, st, prm, stims, 0, n_steps, frame_every, 0.01, 0.01)
+
+    def fwd_bwd():
+        D.grad = None
+        fr = adjoint.rollout(D, *taus, st, prm, stims, 0, n_steps, frame_every, 0.01, 0.01)
+        ((fr - target) ** 2).mean().backward()
+
+    fwd_only(), fwd_bwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fwd_only()
+    torch.cuda.synchronize()
+    tf = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / reps
+    cells = size * size * n_steps
+    return {"workload": "%dx%d, %d Euler steps, u frames every %d, MSE vs a hidden scar D-map, "
+                        "grad wrt D and tau_d/tau_r/tau_si, sqrt(N) checkpoints"
+                        % (size, size, n_steps, frame_every),
+            "forward_s": round(tf, 4), "fwd_adjoint_s": round(tb, 4),
+            "cost_ratio": round(tb / tf, 2),
+            "forward_gcups": round(cells / tf / 1e9, 3),
+            "fwd_adjoint_gcups": round(cells / tb / 1e9, 3),
+            "grad_finite": bool(torch.isfinite(D.grad).all().item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +180,7 @@ def main():
     ap.add_argument("--batch-sims", type=int, default=256,
                     help="simulations of the batched workload (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -197,6 +253,11 @@ def main():
         log("batched workload ...")
         extra["batched"] = bench_batched(args.batch_sims, args.batch_steps, args.seed)
         log("batched: %s" % json.dumps(extra["batched"]))
+
+    if not args.no_adjoint:
+        log("adjoint workload ...")
+        extra["adjoint"] = bench_adjoint(args.seed)
+        log("adjoint: %s" % json.dumps(extra["adjoint"]))
 
     cpu = None
     if not args.no_cpu:

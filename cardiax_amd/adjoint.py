@@ -1,0 +1,130 @@
+"""Differentiable forward-Euler rollout: gradients of a loss on sampled u
+frames with respect to the diffusivity map D and tau_d, tau_r, tau_si.
+
+``rollout(...)`` runs cardiax/solve.py:92-100 on the GPU and returns the u
+fields after every ``frame_every`` steps. Its backward pass is the
+hand-written adjoint of the Euler step (fk_adjoint_step): the states are
+recomputed segment by segment from checkpoints kept every ~sqrt(N) steps, so
+memory is O(sqrt(N)) states instead of N. Gates are fixed masks and the
+stimulus replacement routes no cotangent to j_ion -- what jax.grad computes
+through the reference's greater_equal and where.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Sequence
+
+import torch
+
+from . import _lib
+from ._device import on_device, require_gpu, stream_handle
+from .params import Params
+from .solve import State, _Stimuli, _scalar
+
+# Params fields that carry gradients, in the order of the returned gradients.
+GRAD_FIELDS = ("tau_d", "tau_r", "tau_si")
+
+
+def _with_taus(params, tau_d, tau_r, tau_si) -> Params:
+    p = Params(*[float(x) for x in params])
+    return p._replace(tau_d=float(tau_d), tau_r=float(tau_r), tau_si=float(tau_si))
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, D, tau_d, tau_r, tau_si, cfg):
+        dev = D.device
+        prm = _with_taus(cfg["params"], tau_d.item(), tau_r.item(), tau_si.item())
+        st = State(*(on_device(x, dev).contiguous() for x in cfg["state0"]))
+        H, W = st.u.shape
+        n, fe, t0 = cfg["n_steps"], cfg["frame_every"], cfg["t0"]
+        S = cfg["checkpoint_every"] or max(1, int(math.ceil(math.sqrt(n))))
+        stims = cfg["stimuli"]
+        from . import solve
+
+        frames, ckpts = [], {0: st}
+        k = 0
+        while k < n:
+            nxt = min(n, (k // S + 1) * S, (k // fe + 1) * fe)
+            st = solve._forward_euler(st, t0 + k, t0 + nxt, prm, D, stims, cfg["dt"], cfg["dx"])
+            k = nxt
+            if k % fe == 0:
+                frames.append(st.u)
+            if k % S == 0 and k < n:
+                ckpts[k] = st
+        ctx.cfg = cfg
+        ctx.prm = prm
+        ctx.S = S
+        ctx.ckpts = ckpts
+        ctx.save_for_backward(D)
+        return torch.stack(frames) if frames else torch.zeros((0, H, W), device=dev)
+
+    @staticmethod
+    def backward(ctx, g_frames):
+        (D,) = ctx.saved_tensors
+        cfg, prm, S = ctx.cfg, ctx.prm, ctx.S
+        dev = D.device
+        H, W = D.shape
+        n, fe, t0, dt, dx = cfg["n_steps"], cfg["frame_every"], cfg["t0"], cfg["dt"], cfg["dx"]
+        L = _lib.lib()
+        stim = _Stimuli(cfg["stimuli"], dev, (H, W))
+        p = _lib.make_params(prm)
+        nb = L.fk_adjoint_workspace_bytes(H, W)
+        work = torch.empty(max(1, nb // 8), dtype=torch.float64, device=dev)
+        s = stream_handle()
+        _lib.check(L.fk_adjoint_prepare(D.data_ptr(), H, W, work.data_ptr(), nb, s))
+        g_frames = g_frames.contiguous().to(torch.float32)
+        lam = [torch.zeros((H, W), dtype=torch.float32, device=dev) for _ in range(3)]
+        nxt = [torch.empty_like(lam[0]) for _ in range(3)]
+        gD = torch.zeros((H, W), dtype=torch.float32, device=dev)
+        for c in sorted(ctx.ckpts, reverse=True):
+            e = min(n, c + S)
+            # recompute x_c .. x_{e-1}
+            traj = [ctx.ckpts[c]]
+            for k in range(c, e - 1):
+                x = traj[-1]
+                y = State(*(torch.empty_like(x.u) for _ in range(3)))
+                _lib.check(L.fk_step_euler(
+                    ctypes.byref(p), D.data_ptr(), x.v.data_ptr(), x.w.data_ptr(),
+                    x.u.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + k), float(dt),
+                    float(dx), y.v.data_ptr(), y.w.data_ptr(), y.u.data_ptr(), s))
+                traj.append(y)
+            for k in range(e - 1, c - 1, -1):
+                if (k + 1) % fe == 0:  # x_{k+1} is frame (k+1)/fe - 1
+                    lam[2].add_(g_frames[(k + 1) // fe - 1])
+                x = traj[k - c]
+                _lib.check(L.fk_adjoint_step(
+                    ctypes.byref(p), D.data_ptr(), x.v.data_ptr(), x.w.data_ptr(),
+                    x.u.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + k), float(dt),
+                    float(dx), lam[0].data_ptr(), lam[1].data_ptr(), lam[2].data_ptr(),
+                    nxt[0].data_ptr(), nxt[1].data_ptr(), nxt[2].data_ptr(), gD.data_ptr(),
+                    work.data_ptr(), nb, s))
+                lam, nxt = nxt, lam
+        gt = torch.empty(3, dtype=torch.float32, device=dev)
+        _lib.check(L.fk_adjoint_tau_grads(work.data_ptr(), H, W, gt.data_ptr(), s))
+        return gD, gt[0], gt[1], gt[2], None
+
+
+def rollout(D: torch.Tensor, tau_d, tau_r, tau_si, state0, params, stimuli: Sequence, t0,
+            n_steps: int, frame_every: int, dt: float, dx: float,
+            checkpoint_every: int | None = None) -> torch.Tensor:
+    """u after every ``frame_every`` Euler steps of cardiax/solve.py:92-100,
+    shape (n_steps // frame_every, H, W), differentiable with respect to D
+    (H x W) and the 0-d tensors tau_d, tau_r, tau_si (which replace those
+    fields of ``params``). Other fields of ``params``, the initial state and
+    the stimuli are constants."""
+    dev = require_gpu("rollout")
+    if int(frame_every) <= 0 or int(n_steps) < 0:
+        raise ValueError("frame_every must be positive and n_steps non-negative")
+    D = on_device(D, dev) if not isinstance(D, torch.Tensor) else D
+    if D.dtype != torch.float32 or D.device != dev:
+        raise ValueError("D must be a float32 tensor on the GPU")
+    taus = []
+    for x in (tau_d, tau_r, tau_si):
+        t = x if isinstance(x, torch.Tensor) else torch.tensor(float(x))
+        taus.append(t.to(device=dev, dtype=torch.float32).reshape(()))
+    cfg = {"params": params, "state0": state0, "stimuli": list(stimuli or []),
+           "t0": _scalar(t0), "n_steps": int(n_steps), "frame_every": int(frame_every),
+           "dt": float(dt), "dx": float(dx), "checkpoint_every": checkpoint_every}
+    return _Rollout.apply(D, taus[0], taus[1], taus[2], cfg)

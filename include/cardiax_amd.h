@@ -126,6 +126,40 @@ int32_t fk_stimulate(double t, const float* X, float* out, int64_t n_cells,
                      const fk_stimulus* stimuli, int32_t n_stimuli, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Adjoint of the Euler step, for gradients with respect to the D-map and
+ * tau_d, tau_r, tau_si. No reference counterpart as a function: it is what
+ * jax.grad through cardiax/solve.py:68-70 computes, with the Heaviside gates
+ * (greater_equal, :35-36) as fixed masks and no cotangent reaching j_ion
+ * where a stimulus replaces it (jnp.where, :271).
+ * ------------------------------------------------------------------------- */
+
+/* Scratch bytes of the adjoint for an H x W grid (tables of the boundary
+ * operators, D_x, D_y and the per-block tau partial sums). */
+size_t fk_adjoint_workspace_bytes(int32_t H, int32_t W);
+
+/* Fills the workspace for D and zeroes the tau accumulators; call once per
+ * backward pass before the fk_adjoint_step calls. */
+int32_t fk_adjoint_prepare(const float* D, int32_t H, int32_t W, void* work,
+                           size_t work_bytes, void* stream);
+
+/* One reverse Euler step at time t: given the state (v, w, u) at that step
+ * and the cotangent lam of the next state, writes the cotangent of this
+ * state to lam_*_out (must not alias the inputs), adds this step's
+ * contribution to grad_D (H x W, accumulated in place) and to the workspace's
+ * tau accumulators. */
+int32_t fk_adjoint_step(const fk_params* params, const float* D, const float* v,
+                        const float* w, const float* u, int32_t H, int32_t W,
+                        const fk_stimulus* stimuli, int32_t n_stimuli, double t, float dt,
+                        float dx, const float* lam_v, const float* lam_w, const float* lam_u,
+                        float* lam_v_out, float* lam_w_out, float* lam_u_out, float* grad_D,
+                        void* work, size_t work_bytes, void* stream);
+
+/* Writes the accumulated gradients with respect to (tau_d, tau_r, tau_si) to
+ * g_tau (3 floats, device memory). */
+int32_t fk_adjoint_tau_grads(const void* work, int32_t H, int32_t W, float* g_tau,
+                             void* stream);
+
+/* ---------------------------------------------------------------------------
  * Batched forward Euler. No reference counterpart: the reference runs one
  * simulation per call (experiments/generate_fd_data_128.py:25-56 loops over
  * seeds). Each simulation follows cardiax/solve.py:92-100 exactly as
