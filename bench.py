@@ -29,6 +29,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel, H, W):
+    """HBM bytes per launch of `kernel` at H x W from the committed rocprofv3
+    --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known
+    size); None when no summary covers this kernel and size."""
+    import glob
+    import re
+
+    key = "traffic_bytes_per_launch[%s %dx%d]" % (kernel, H, W)
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key):
+                    m = re.search(r"=\s*(\d+)", line)
+                    if m:
+                        return int(m.group(1))
+    return None
+
+
 def cpu_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -175,8 +193,11 @@ def main():
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain", action="store_true",
-                    help="also time the plain one-thread-per-cell kernel")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip timing the plain one-thread-per-cell fp32 kernel")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run torch's copy kernel over MIB MiB 20 times (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     ap.add_argument("--batch-sims", type=int, default=256,
                     help="simulations of the batched workload (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=5000)
@@ -195,6 +216,15 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        n = args.copy_calibration << 18
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "bytes_written": 4 * n}))
+        return
     H = W = args.size
     sim = workloads.c2_inputs(args.seed, shape=(H, W), steps=args.sim_steps)
     stims = [(s.protocol, torch.from_numpy(s.field).to(dev)) for s in sim.stimuli]
@@ -230,16 +260,17 @@ def main():
 
     # Dominant kernel: one launch of the tiled step kernel per Euler step.
     launches = sim.t1 * args.steps
+    kernel = "k_stream<8, true>"
     per_launch_s = (ev_ms / 1e3) / launches
     achieved = BYTES_PER_CELL_UPDATE * H * W / per_launch_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_tile<32,true>", "per_launch_us": round(per_launch_s * 1e6, 3),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(kernel, H, W), "kernel": kernel, "per_launch_us": round(per_launch_s * 1e6, 3),
                 "bytes_per_launch": BYTES_PER_CELL_UPDATE * H * W}
 
     extra = {"event_gcups": round(cell_updates / (ev_ms / 1e3) / 1e9, 3),
              "u_finite": finite, "u_absmax": round(umax, 4)}
-    if args.plain:
+    if not args.no_plain:
         n = 2000
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -247,7 +278,9 @@ def main():
         solve._forward_euler_plain(st0, 0, n, sim.params, D, stims, sim.dt, sim.dx)
         e1.record(stream)
         torch.cuda.synchronize()
-        extra["plain_kernel_gcups"] = round(H * W * n / (e0.elapsed_time(e1) / 1e3) / 1e9, 3)
+        plain = H * W * n / (e0.elapsed_time(e1) / 1e3) / 1e9
+        extra["plain_fp32_kernel_gcups"] = round(plain, 3)
+        extra["speedup_vs_plain_fp32"] = round(extra["event_gcups"] / plain, 3)
 
     if args.batch_sims > 0:
         log("batched workload ...")
@@ -266,7 +299,7 @@ def main():
         log("cpu baseline: %.4f GCUPS on %d cores" % (cpu["value"], cpu["cores"]))
 
     rec = {
-        "metric": "GCUPS (giga cell-updates/s), fused forward-Euler FK step, fp32",
+        "metric": "GCUPS + % HBM roofline (fwd, batched, fwd+adjoint) vs plain fp32 at 1 GPU",
         "value": round(gcups, 3),
         "unit": "GCUPS",
         "n_gpus": 1,
