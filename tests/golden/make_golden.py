@@ -41,6 +41,12 @@ def case(name, shape, pname, D, stimuli, steps, checkpoints, seed, init="rest"):
     else:  # a perturbed, partly excited state
         st = o.State(rng.uniform(0.3, 1.0, shape), rng.uniform(0.3, 1.0, shape),
                      rng.uniform(0.0, 0.9, shape))
+    # the fixture stores float32 inputs, so the FP64 run starts from exactly
+    # those values
+    st = o.State(*(np.asarray(x, np.float32).astype(np.float64) for x in st))
+    D = np.asarray(D, np.float32).astype(np.float64)
+    stimuli = [o.Stimulus(s.protocol, np.asarray(s.field, np.float32).astype(np.float64))
+               for s in stimuli]
     prm = P.ALL_PARAMSETS[pname]
     outs = []
     cur, t = st, 0
