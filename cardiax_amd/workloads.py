@@ -69,7 +69,8 @@ def scar_map(shape, rng, n_min=1, n_max=5, axes=(20, 200), scale=(0.1, 0.2), bas
     U[scale], with a smoothstep edge max(U[taper], taper_frac * side) cells
     wide. The edge scales with the grid like the reference's blur, whose
     kernel spans 10% of the image (deepx/utils_scars.py:402-426), and the
-    scale stays >= 0.1, the floor of the reference's own random D range
+    scale stays >= 0.1 (overlaps take the minimum, not the product), the floor
+    of the reference's own random D range
     [1e-4, 1e-3] (deepx/generate.py:80). Both keep the dynamics bounded: the
     reference's non-conservative D (u_xx + u_yy) + D_x u_x + D_y u_y with its
     stencils has growing modes where D changes steeply (edges of 3% of the
@@ -95,7 +96,9 @@ def scar_map(shape, rng, n_min=1, n_max=5, axes=(20, 200), scale=(0.1, 0.2), bas
         dist = (1.0 - r) * min(a, b)  # ~ signed distance inside the ellipse, in cells
         x = np.clip(dist / tw + 0.5, 0.0, 1.0)
         inside = x * x * (3 - 2 * x)
-        factor *= 1.0 - (1.0 - f) * inside
+        # overlapping scars take the deeper one, so D never leaves
+        # [scale[0] * base, base]
+        factor = np.minimum(factor, 1.0 - (1.0 - f) * inside)
     return (base * factor).astype(np.float32)
 
 
