@@ -74,6 +74,9 @@ _SIGNATURES = {
     "fk_forward_euler_plain": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp,
                                        c_i32, c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d,
                                        c_d, c_f, c_f, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "fk_forward_trajectory": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32,
+                                      c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d, c_i32,
+                                      c_f, c_f, c_vp, c_vp]),
     "fk_gradient": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp]),
     "fk_stimulate": (c_i32, [c_d, c_vp, c_vp, c_i64, ctypes.POINTER(FkStimulus), c_i32, c_vp]),
     "fk_adjoint_workspace_bytes": (c_sz, [c_i32, c_i32]),
@@ -81,6 +84,10 @@ _SIGNATURES = {
     "fk_adjoint_step": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
                                 ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_f, c_vp, c_vp,
                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "fk_adjoint_segment": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_i32, c_i32,
+                                   ctypes.POINTER(FkStimulus), c_i32, c_d, c_i32, c_f, c_f,
+                                   c_vp, ctypes.POINTER(c_vp), c_vp, c_vp, c_vp, c_vp, c_vp,
+                                   c_sz, c_vp]),
     "fk_adjoint_tau_grads": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_vp]),
     "fk_batch_create": (c_i32, [ctypes.POINTER(FkSim), c_i32, ctypes.POINTER(c_vp)]),
     "fk_batch_run": (c_i32, [c_vp, c_i64, c_i32, c_vp]),

@@ -3,9 +3,10 @@ frames with respect to the diffusivity map D and tau_d, tau_r, tau_si.
 
 ``rollout(...)`` runs cardiax/solve.py:92-100 on the GPU and returns the u
 fields after every ``frame_every`` steps. Its backward pass is the
-hand-written adjoint of the Euler step (fk_adjoint_step): the states are
-recomputed segment by segment from checkpoints kept every ~sqrt(N) steps, so
-memory is O(sqrt(N)) states instead of N. Gates are fixed masks and the
+hand-written adjoint of the Euler step: the states are recomputed segment by
+segment from checkpoints kept every ~sqrt(N) steps (fk_forward_trajectory),
+so memory is O(sqrt(N)) states instead of N, and each segment is reversed by
+one fk_adjoint_segment call. Gates are fixed masks and the
 stimulus replacement routes no cotangent to j_ion -- what jax.grad computes
 through the reference's greater_equal and where.
 """
@@ -40,25 +41,36 @@ class _Rollout(torch.autograd.Function):
         H, W = st.u.shape
         n, fe, t0 = cfg["n_steps"], cfg["frame_every"], cfg["t0"]
         S = cfg["checkpoint_every"] or max(1, int(math.ceil(math.sqrt(n))))
-        stims = cfg["stimuli"]
-        from . import solve
-
-        frames, ckpts = [], {0: st}
+        L = _lib.lib()
+        stim = _Stimuli(cfg["stimuli"], dev, (H, W))
+        p = _lib.make_params(prm)
+        nb = L.fk_forward_workspace_bytes(H, W)
+        work = torch.empty(max(1, nb // 4), dtype=torch.float32, device=dev)
+        s = stream_handle()
+        frames = torch.empty((n // fe, H, W), dtype=torch.float32, device=dev)
+        ckpts = {0: st}
+        cur = st
         k = 0
         while k < n:
             nxt = min(n, (k // S + 1) * S, (k // fe + 1) * fe)
-            st = solve._forward_euler(st, t0 + k, t0 + nxt, prm, D, stims, cfg["dt"], cfg["dx"])
-            k = nxt
+            out = State(*(torch.empty((H, W), dtype=torch.float32, device=dev)
+                          for _ in range(3)))
+            _lib.check(L.fk_forward_euler(
+                ctypes.byref(p), D.data_ptr(), cur.v.data_ptr(), cur.w.data_ptr(),
+                cur.u.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + k), float(t0 + nxt),
+                float(cfg["dt"]), float(cfg["dx"]), out.v.data_ptr(), out.w.data_ptr(),
+                out.u.data_ptr(), work.data_ptr(), nb, s))
+            cur, k = out, nxt
             if k % fe == 0:
-                frames.append(st.u)
+                frames[k // fe - 1].copy_(cur.u)
             if k % S == 0 and k < n:
-                ckpts[k] = st
+                ckpts[k] = cur
         ctx.cfg = cfg
         ctx.prm = prm
         ctx.S = S
         ctx.ckpts = ckpts
         ctx.save_for_backward(D)
-        return torch.stack(frames) if frames else torch.zeros((0, H, W), device=dev)
+        return frames
 
     @staticmethod
     def backward(ctx, g_frames):
@@ -76,31 +88,29 @@ class _Rollout(torch.autograd.Function):
         _lib.check(L.fk_adjoint_prepare(D.data_ptr(), H, W, work.data_ptr(), nb, s))
         g_frames = g_frames.contiguous().to(torch.float32)
         lam = [torch.zeros((H, W), dtype=torch.float32, device=dev) for _ in range(3)]
-        nxt = [torch.empty_like(lam[0]) for _ in range(3)]
         gD = torch.zeros((H, W), dtype=torch.float32, device=dev)
+        # states x_c .. x_{e-1} of one segment, (v, w, u) planes per state
+        states = torch.empty((S, 3, H, W), dtype=torch.float32, device=dev)
         for c in sorted(ctx.ckpts, reverse=True):
             e = min(n, c + S)
-            # recompute x_c .. x_{e-1}
-            traj = [ctx.ckpts[c]]
-            for k in range(c, e - 1):
-                x = traj[-1]
-                y = State(*(torch.empty_like(x.u) for _ in range(3)))
-                _lib.check(L.fk_step_euler(
+            x = ctx.ckpts[c]
+            states[0, 0].copy_(x.v)
+            states[0, 1].copy_(x.w)
+            states[0, 2].copy_(x.u)
+            if e - c > 1:  # recompute x_{c+1} .. x_{e-1}
+                _lib.check(L.fk_forward_trajectory(
                     ctypes.byref(p), D.data_ptr(), x.v.data_ptr(), x.w.data_ptr(),
-                    x.u.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + k), float(dt),
-                    float(dx), y.v.data_ptr(), y.w.data_ptr(), y.u.data_ptr(), s))
-                traj.append(y)
-            for k in range(e - 1, c - 1, -1):
-                if (k + 1) % fe == 0:  # x_{k+1} is frame (k+1)/fe - 1
-                    lam[2].add_(g_frames[(k + 1) // fe - 1])
-                x = traj[k - c]
-                _lib.check(L.fk_adjoint_step(
-                    ctypes.byref(p), D.data_ptr(), x.v.data_ptr(), x.w.data_ptr(),
-                    x.u.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + k), float(dt),
-                    float(dx), lam[0].data_ptr(), lam[1].data_ptr(), lam[2].data_ptr(),
-                    nxt[0].data_ptr(), nxt[1].data_ptr(), nxt[2].data_ptr(), gD.data_ptr(),
-                    work.data_ptr(), nb, s))
-                lam, nxt = nxt, lam
+                    x.u.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + c), e - c - 1,
+                    float(dt), float(dx), states[1].data_ptr(), s))
+            # frame cotangents enter lam_u of x_{k+1} before step k is reversed
+            inject = (ctypes.c_void_p * (e - c))()
+            for k in range(c, e):
+                if (k + 1) % fe == 0:
+                    inject[k - c] = g_frames[(k + 1) // fe - 1].data_ptr()
+            _lib.check(L.fk_adjoint_segment(
+                ctypes.byref(p), D.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + c),
+                e - c, float(dt), float(dx), states.data_ptr(), inject, lam[0].data_ptr(),
+                lam[1].data_ptr(), lam[2].data_ptr(), gD.data_ptr(), work.data_ptr(), nb, s))
         gt = torch.empty(3, dtype=torch.float32, device=dev)
         _lib.check(L.fk_adjoint_tau_grads(work.data_ptr(), H, W, gt.data_ptr(), s))
         return gD, gt[0], gt[1], gt[2], None

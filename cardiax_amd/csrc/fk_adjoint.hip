@@ -141,27 +141,166 @@ __global__ void k_adj_dgrad(const float* __restrict__ D, int H, int W,
 
 constexpr int kAdjThreads = 256;
 
-// One reverse step. Block partial sums of the tau gradients are added to
-// g_tau_part[block][0..2] (each block owns its slot: deterministic).
+// Central stencils of the untabulated rows: A1 = G (d = -2..2) and A2 = G G
+// (d = -4..4).
+#define FK_A1(d) ((d) == -2 ? 1.0f / 12.0f : (d) == -1 ? -2.0f / 3.0f : (d) == 1 ? 2.0f / 3.0f : (d) == 2 ? -1.0f / 12.0f : 0.0f)
+#define FK_A2(d)                                                                          \
+  ((d) == 0 ? -65.0f / 72.0f                                                              \
+            : ((d) == 1 || (d) == -1) ? 1.0f / 9.0f                                       \
+            : ((d) == 2 || (d) == -2) ? 4.0f / 9.0f                                       \
+            : ((d) == 3 || (d) == -3) ? -1.0f / 9.0f                                      \
+            : ((d) == 4 || (d) == -4) ? 1.0f / 144.0f : 0.0f)
+
+// A1[j][j + d] and A2[j][j + d] without branches on the memory path: the
+// table entry is always loaded (slot clamped into range) and then selected,
+// so unrolled loops issue every load up front. Zero when j is off the axis.
+__device__ __forceinline__ void band_coef(const AdjAxisTab& T, int j, int d, int N, float& a1,
+                                          float& a2) {
+  const bool on = j >= 0 && j < N;
+  const int s = adj_slot(on ? j : 0, N);
+  const int sc = s < 0 ? 0 : s;
+  const float t1 = T.a1[sc][d + kAdjBand], t2 = T.a2[sc][d + kAdjBand];
+  a1 = !on ? 0.0f : (s >= 0 ? t1 : FK_A1(d));
+  a2 = !on ? 0.0f : (s >= 0 ? t2 : FK_A2(d));
+}
+
+// Row j of an axis of N cells uses the central stencils (not a table row).
+__device__ __forceinline__ bool central_row(int j, int N) { return adj_slot(j, N) < 0; }
+// Every row within the band of j is central, so (A^T q)(j) uses the
+// transposed central stencils only.
+__device__ __forceinline__ bool central_cone(int j, int N) {
+  return N > kAdjSlots && j >= kAdjB + kAdjBand && j < N - kAdjB - kAdjBand;
+}
+
+// Fields the reverse step derives from u and the incoming cotangent before
+// applying the transposed operators: L = lam_u (+ injected cotangent),
+// Q2 = D L, Q1x = D_x L, Q1y = D_y L, Px = L (A1x u), Py = L (A1y u), and
+// lap = A2x u + A2y u.
+struct AdjPre {
+  float *L, *Q2, *Q1x, *Q1y, *Px, *Py, *lap;
+};
+
+// Inputs of the derived fields of one cell: u at offsets -5..5 along both
+// axes (clamped into the grid; the coefficients there are 0), D, D_x, D_y.
+struct PreIn {
+  float xs[2 * kAdjBand + 1], ys[2 * kAdjBand + 1];
+  float D, Dx, Dy;
+};
+
+__device__ __forceinline__ void pre_load(const float* __restrict__ u,
+                                         const float* __restrict__ D,
+                                         const float* __restrict__ Dx,
+                                         const float* __restrict__ Dy, int i, int c, int H,
+                                         int W, PreIn& in) {
+  const size_t idx = (size_t)i * W + c;
+#pragma unroll
+  for (int d = -kAdjBand; d <= kAdjBand; ++d) {
+    in.xs[d + kAdjBand] = u[(size_t)clampi(i + d, 0, H - 1) * W + c];
+    in.ys[d + kAdjBand] = u[(size_t)i * W + clampi(c + d, 0, W - 1)];
+  }
+  in.D = D[idx];
+  in.Dx = Dx[idx];
+  in.Dy = Dy[idx];
+}
+
+// The derived fields of one cell, given its cotangent L (lam_u plus any
+// injected cotangent) and the loaded inputs.
+__device__ __forceinline__ void pre_store(const PreIn& in, float L, int i, int c, int H, int W,
+                                          const AdjAxisTab* __restrict__ tab,
+                                          const AdjPre& o) {
+  const size_t idx = (size_t)i * W + c;
+  float gx = 0.0f, gy = 0.0f, lap = 0.0f;
+  if (central_row(i, H)) {
+#pragma unroll
+    for (int d = -4; d <= 4; ++d) {
+      lap += FK_A2(d) * in.xs[d + kAdjBand];
+      gx += FK_A1(d) * in.xs[d + kAdjBand];
+    }
+  } else {
+#pragma unroll
+    for (int d = -kAdjBand; d <= kAdjBand; ++d) {
+      float a1, a2;
+      band_coef(tab[0], i, d, H, a1, a2);
+      const bool on = i + d >= 0 && i + d < H;
+      gx += on ? a1 * in.xs[d + kAdjBand] : 0.0f;
+      lap += on ? a2 * in.xs[d + kAdjBand] : 0.0f;
+    }
+  }
+  if (central_row(c, W)) {
+#pragma unroll
+    for (int d = -4; d <= 4; ++d) {
+      lap += FK_A2(d) * in.ys[d + kAdjBand];
+      gy += FK_A1(d) * in.ys[d + kAdjBand];
+    }
+  } else {
+#pragma unroll
+    for (int d = -kAdjBand; d <= kAdjBand; ++d) {
+      float a1, a2;
+      band_coef(tab[1], c, d, W, a1, a2);
+      const bool on = c + d >= 0 && c + d < W;
+      gy += on ? a1 * in.ys[d + kAdjBand] : 0.0f;
+      lap += on ? a2 * in.ys[d + kAdjBand] : 0.0f;
+    }
+  }
+  o.L[idx] = L;
+  o.Q2[idx] = in.D * L;
+  o.Q1x[idx] = in.Dx * L;
+  o.Q1y[idx] = in.Dy * L;
+  o.Px[idx] = L * gx;
+  o.Py[idx] = L * gy;
+  o.lap[idx] = lap;
+}
+
 __global__ void __launch_bounds__(kAdjThreads)
-k_adjoint_step(Coef c, const float* __restrict__ D, const float* __restrict__ Dx,
-               const float* __restrict__ Dy, const float* __restrict__ v,
-               const float* __restrict__ w, const float* __restrict__ u, int H, int W,
-               StimArgs sa, float dt, float inv_dx2, const float* __restrict__ lv,
-               const float* __restrict__ lw, const float* __restrict__ lu,
-               float* __restrict__ lv_o, float* __restrict__ lw_o, float* __restrict__ lu_o,
-               float* __restrict__ gD, double* __restrict__ g_tau_part,
-               const AdjAxisTab* __restrict__ tab) {
-  const int c_ = blockIdx.x * 64 + (threadIdx.x & 63);
+k_adj_pre(const float* __restrict__ u, const float* __restrict__ D,
+          const float* __restrict__ Dx, const float* __restrict__ Dy,
+          const float* __restrict__ lu, const float* __restrict__ inj, int H, int W,
+          const AdjAxisTab* __restrict__ tab, AdjPre o) {
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  float tdl = 0.0f, trl = 0.0f, tsl = 0.0f;
-  if (i < H && c_ < W) {
-    const int col = c_;
+  if (i >= H || c >= W) return;
+  const size_t idx = (size_t)i * W + c;
+  PreIn in;
+  pre_load(u, D, Dx, Dy, i, c, H, W, in);
+  pre_store(in, inj ? lu[idx] + inj[idx] : lu[idx], i, c, H, W, tab, o);
+}
+
+// What the reverse step of t+k-1 needs from the step of t+k: the state
+// x_{t+k-1}'s u and the cotangent injected into x_{t+k} (may be null).
+struct AdjNext {
+  const float* u_prev;
+  const float* inj;
+  const float *D, *Dx, *Dy;
+  AdjPre pre;
+};
+
+// One reverse step from the derived fields. Each cell adds its contribution
+// to the tau gradients to its own entry of the three accumulator fields
+// g_tau[k * H * W + cell] (deterministic; summed by k_adj_reduce).
+// NEXT: also write the derived fields of the previous step's reverse (they
+// depend on the new cotangent only at the same cell), so a segment needs one
+// launch per step.
+template <bool NEXT>
+__global__ void __launch_bounds__(kAdjThreads)
+k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
+           const float* __restrict__ u, int H, int W, StimArgs sa, float dt, float inv_dx2,
+           const float* __restrict__ lv, const float* __restrict__ lw, AdjPre pre,
+           float* __restrict__ lv_o, float* __restrict__ lw_o, float* __restrict__ lu_o,
+           float* __restrict__ gD, float* __restrict__ g_tau,
+           const AdjAxisTab* __restrict__ tab, AdjNext nx) {
+  const int col = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i < H && col < W) {
     const size_t idx = (size_t)i * W + col;
-    const AxisOps ox{&tab[0], H}, oy{&tab[1], W};
-    auto at = [&](const float* a, int r, int q) { return a[(size_t)r * W + q]; };
+    const size_t n = (size_t)H * W;
+    PreIn next{};
+    float inj = 0.0f;
+    if (NEXT) {  // issued first: independent of everything below
+      pre_load(nx.u_prev, nx.D, nx.Dx, nx.Dy, i, col, H, W, next);
+      if (nx.inj) inj = nx.inj[idx];
+    }
     const float vv = v[idx], ww = w[idx], uu = u[idx];
-    const float lvv = lv[idx], lww = lw[idx], luu = lu[idx];
+    const float lvv = lv[idx], lww = lw[idx], L = pre.L[idx];
 
     // --- reaction, gates as fixed masks ---------------------------------
     const bool p = uu >= c.V_c;
@@ -181,93 +320,85 @@ k_adjoint_step(Coef c, const float* __restrict__ D, const float* __restrict__ Dx
     // stimulus replaces j_ion: no cotangent reaches it where active
     bool stim = false;
     for (int k = 0; k < sa.n; ++k) stim = stim || (sa.f[k][idx] != 0.0f);
-    const float lus = stim ? 0.0f : luu;
+    const float lus = stim ? 0.0f : L;
 
-    // --- diffusion transposes --------------------------------------------
-    float sx = 0.0f, sy = 0.0f;  // A2^T(D lam) + A1^T(D_? lam)
-    float ex = 0.0f, ey = 0.0f;  // A1^T(lam A1 u)
-    float lap = 0.0f;            // A2x u + A2y u at the cell
-    for (int d = -kAdjBand; d <= kAdjBand; ++d) {
-      const int jr = i - d;  // row whose entry [jr][i] is needed: A[jr][jr + d]
-      if (jr >= 0 && jr < H) {
-        const float a2 = ox.a2(jr, d), a1 = ox.a1(jr, d);
-        if (a2 != 0.0f || a1 != 0.0f) {
-          const float lam = at(lu, jr, col);
-          sx += a2 * at(D, jr, col) * lam + a1 * at(Dx, jr, col) * lam;
-          if (a1 != 0.0f) {
-            float uxj = 0.0f;
-            for (int e = -kAdjBand; e <= kAdjBand; ++e)
-              if (jr + e >= 0 && jr + e < H) uxj += ox.a1(jr, e) * at(u, jr + e, col);
-            ex += a1 * lam * uxj;
-          }
+    // --- transposed diffusion operators: (A^T q)(i) = sum_d A[i-d][i] q(i-d)
+    float sx = 0.0f, sy = 0.0f;  // A2^T(D L) + A1^T(D_? L)
+    float ex = 0.0f, ey = 0.0f;  // A1^T(L A1 u)
+    if (central_cone(i, H)) {
+#pragma unroll
+      for (int d = -4; d <= 4; ++d) {
+        const size_t j = (size_t)(i - d) * W + col;
+        sx += FK_A2(d) * pre.Q2[j];
+        if (d >= -2 && d <= 2 && d != 0) {
+          sx += FK_A1(d) * pre.Q1x[j];
+          ex += FK_A1(d) * pre.Px[j];
         }
       }
-      const int jc = col - d;
-      if (jc >= 0 && jc < W) {
-        const float a2 = oy.a2(jc, d), a1 = oy.a1(jc, d);
-        if (a2 != 0.0f || a1 != 0.0f) {
-          const float lam = at(lu, i, jc);
-          sy += a2 * at(D, i, jc) * lam + a1 * at(Dy, i, jc) * lam;
-          if (a1 != 0.0f) {
-            float uyj = 0.0f;
-            for (int e = -kAdjBand; e <= kAdjBand; ++e)
-              if (jc + e >= 0 && jc + e < W) uyj += oy.a1(jc, e) * at(u, i, jc + e);
-            ey += a1 * lam * uyj;
-          }
+    } else {
+#pragma unroll
+      for (int d = -kAdjBand; d <= kAdjBand; ++d) {
+        const int jr = i - d;
+        float a1, a2;
+        band_coef(tab[0], jr, d, H, a1, a2);  // 0 when jr is off the axis
+        const size_t j = (size_t)clampi(jr, 0, H - 1) * W + col;
+        sx += a2 * pre.Q2[j] + a1 * pre.Q1x[j];
+        ex += a1 * pre.Px[j];
+      }
+    }
+    if (central_cone(col, W)) {
+#pragma unroll
+      for (int d = -4; d <= 4; ++d) {
+        const size_t j = idx - d;
+        sy += FK_A2(d) * pre.Q2[j];
+        if (d >= -2 && d <= 2 && d != 0) {
+          sy += FK_A1(d) * pre.Q1y[j];
+          ey += FK_A1(d) * pre.Py[j];
         }
       }
-      if (i + d >= 0 && i + d < H) lap += ox.a2(i, d) * at(u, i + d, col);
-      if (col + d >= 0 && col + d < W) lap += oy.a2(col, d) * at(u, i, col + d);
+    } else {
+#pragma unroll
+      for (int d = -kAdjBand; d <= kAdjBand; ++d) {
+        const int jc = col - d;
+        float a1, a2;
+        band_coef(tab[1], jc, d, W, a1, a2);  // 0 when jc is off the axis
+        const size_t j = (size_t)i * W + clampi(jc, 0, W - 1);
+        sy += a2 * pre.Q2[j] + a1 * pre.Q1y[j];
+        ey += a1 * pre.Py[j];
+      }
     }
 
     lv_o[idx] = lvv + dt * (dfv_dv * lvv + djdv * lus);
     lw_o[idx] = lww + dt * (dfw_dw * lww + djdw * lus);
-    lu_o[idx] = luu + dt * (djdu * lus + (sx + sy) * inv_dx2);
-    gD[idx] += dt * (luu * lap + ex + ey) * inv_dx2;
+    const float lu_new = L + dt * (djdu * lus + (sx + sy) * inv_dx2);
+    lu_o[idx] = lu_new;
+    if (NEXT) pre_store(next, lu_new + inj, i, col, H, W, tab, nx.pre);
+    gD[idx] += dt * (L * pre.lap[idx] + ex + ey) * inv_dx2;
     // tau gradients: d j_ion / d tau
     const float djd_td = p ? -vv * (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_td * c.inv_cm
                            : 0.0f;
     const float djd_tr = p ? c.inv_tr * c.inv_tr * c.inv_cm : 0.0f;
     const float djd_tsi = -ww * S * c.inv_tsi * c.inv_tsi * c.inv_cm;
-    tdl = dt * djd_td * lus;
-    trl = dt * djd_tr * lus;
-    tsl = dt * djd_tsi * lus;
-  }
-  // block reduction (wave shuffles, then LDS across the 4 waves)
-  __shared__ float red[3][kAdjThreads / 64];
-  for (int o = 32; o > 0; o >>= 1) {
-    tdl += __shfl_xor(tdl, o);
-    trl += __shfl_xor(trl, o);
-    tsl += __shfl_xor(tsl, o);
-  }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[0][wv] = tdl;
-    red[1][wv] = trl;
-    red[2][wv] = tsl;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int b = blockIdx.y * gridDim.x + blockIdx.x;
-    for (int k = 0; k < 3; ++k) {
-      double s = 0.0;
-      for (int q = 0; q < kAdjThreads / 64; ++q) s += (double)red[k][q];
-      g_tau_part[3 * (size_t)b + k] += s;
-    }
+    // per-cell accumulators of the tau gradients (reduced once at the end)
+    g_tau[idx] += dt * djd_td * lus;
+    g_tau[n + idx] += dt * djd_tr * lus;
+    g_tau[2 * n + idx] += dt * djd_tsi * lus;
   }
 }
 
-__global__ void k_adj_zero(double* a, size_t n) {
+__global__ void k_adj_zero(float* a, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
-    a[i] = 0.0;
+    a[i] = 0.0f;
 }
 
-__global__ void k_adj_reduce(const double* __restrict__ part, size_t n_blocks, float* g_tau) {
+// Sum of each of the three per-cell tau accumulators (n cells each), in
+// double.
+__global__ void k_adj_reduce(const float* __restrict__ part, size_t n, float* g_tau) {
   __shared__ double s[3][256];
   for (int k = 0; k < 3; ++k) {
     double acc = 0.0;
-    for (size_t b = threadIdx.x; b < n_blocks; b += blockDim.x) acc += part[3 * b + k];
+    for (size_t b = threadIdx.x; b < n; b += blockDim.x) acc += (double)part[k * n + b];
     s[k][threadIdx.x] = acc;
   }
   __syncthreads();
@@ -280,16 +411,15 @@ __global__ void k_adj_reduce(const double* __restrict__ part, size_t n_blocks, f
     for (int k = 0; k < 3; ++k) g_tau[k] = (float)s[k][0];
 }
 
-// Workspace layout: [tables (2 axes)][Dx][Dy][tau partials]
+// Workspace layout: [tables (2 axes)][Dx][Dy][tau accumulators: 3 fields][cotangent
+// ping-pong: 3 fields][derived fields: 2 sets of 7]
 struct AdjLayout {
-  size_t tab, dx, dy, part, total, n_blocks;
+  size_t tab, dx, dy, part, lam, pre, total;
 };
 
 static AdjLayout adj_layout(int H, int W) {
   AdjLayout L;
   const size_t cells = (size_t)H * W;
-  const dim3 grid((W + 63) / 64, (H + 3) / 4);
-  L.n_blocks = (size_t)grid.x * grid.y;
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
@@ -299,9 +429,52 @@ static AdjLayout adj_layout(int H, int W) {
   L.tab = take(2 * sizeof(AdjAxisTab));
   L.dx = take(cells * sizeof(float));
   L.dy = take(cells * sizeof(float));
-  L.part = take(3 * L.n_blocks * sizeof(double));
+  L.part = take(3 * cells * sizeof(float));
+  L.lam = take(3 * cells * sizeof(float));
+  L.pre = take(2 * 7 * cells * sizeof(float));
   L.total = o;
   return L;
+}
+
+static AdjPre pre_fields(const AdjLayout& L, char* wk, int set, size_t n) {
+  float* pf = (float*)(wk + L.pre) + (size_t)set * 7 * n;
+  return AdjPre{pf, pf + n, pf + 2 * n, pf + 3 * n, pf + 4 * n, pf + 5 * n, pf + 6 * n};
+}
+
+static hipError_t launch_pre(const AdjLayout& L, char* wk, const float* D, const float* u,
+                             int H, int W, const float* lu, const float* inj, int set,
+                             hipStream_t s) {
+  const size_t n = (size_t)H * W;
+  const dim3 grid((W + 63) / 64, (H + 3) / 4);
+  k_adj_pre<<<grid, kAdjThreads, 0, s>>>(u, D, (const float*)(wk + L.dx),
+                                         (const float*)(wk + L.dy), lu, inj, H, W,
+                                         (const AdjAxisTab*)(wk + L.tab),
+                                         pre_fields(L, wk, set, n));
+  return hipGetLastError();
+}
+
+// Reverse step from derived-field set `set`: cotangents lam_in (v, w; u comes
+// from the fields) -> lam_out. With u_prev, also writes set 1 - set for the
+// step before (u_prev = that step's state u, inj = its injected cotangent).
+static hipError_t launch_main(const AdjLayout& L, char* wk, const Coef& c, const float* D,
+                              const float* v, const float* w, const float* u, int H, int W,
+                              const StimArgs& sa, float dt, float inv_dx2,
+                              const float* const lam_in[3], float* const lam_out[3], float* gD,
+                              int set, const float* u_prev, const float* inj, hipStream_t s) {
+  const size_t n = (size_t)H * W;
+  const AdjAxisTab* tab = (const AdjAxisTab*)(wk + L.tab);
+  const dim3 grid((W + 63) / 64, (H + 3) / 4);
+  const AdjNext nx{u_prev, inj, D, (const float*)(wk + L.dx), (const float*)(wk + L.dy),
+                   pre_fields(L, wk, 1 - set, n)};
+  if (u_prev)
+    k_adj_main<true><<<grid, kAdjThreads, 0, s>>>(
+        c, v, w, u, H, W, sa, dt, inv_dx2, lam_in[0], lam_in[1], pre_fields(L, wk, set, n),
+        lam_out[0], lam_out[1], lam_out[2], gD, (float*)(wk + L.part), tab, nx);
+  else
+    k_adj_main<false><<<grid, kAdjThreads, 0, s>>>(
+        c, v, w, u, H, W, sa, dt, inv_dx2, lam_in[0], lam_in[1], pre_fields(L, wk, set, n),
+        lam_out[0], lam_out[1], lam_out[2], gD, (float*)(wk + L.part), tab, nx);
+  return hipGetLastError();
 }
 
 }  // namespace fk
@@ -332,7 +505,7 @@ int32_t fk_adjoint_prepare(const float* D, int32_t H, int32_t W, void* work, siz
   const dim3 block(64, 4), grid((W + 63) / 64, (H + 3) / 4);
   k_adj_dgrad<<<grid, block, 0, s>>>(D, H, W, tab, (float*)(wk + L.dx), (float*)(wk + L.dy));
   HIP_CHECK(hipGetLastError());
-  k_adj_zero<<<64, 256, 0, s>>>((double*)(wk + L.part), 3 * L.n_blocks);
+  k_adj_zero<<<256, 256, 0, s>>>((float*)(wk + L.part), 3 * (size_t)H * W);
   HIP_CHECK(hipGetLastError());
   return FK_OK;
 }
@@ -353,16 +526,65 @@ int32_t fk_adjoint_step(const fk_params* params, const float* D, const float* v,
   if (work_bytes < L.total)
     return set_error(FK_ERR_ARG, "adjoint workspace too small: need %zu bytes, got %zu",
                      L.total, work_bytes);
-  char* wk = (char*)work;
   StimSchedule sched(stimuli, n_stimuli);
   const Coef c = make_coef(*params);
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
-  const dim3 grid((W + 63) / 64, (H + 3) / 4);
-  k_adjoint_step<<<grid, kAdjThreads, 0, (hipStream_t)stream>>>(
-      c, D, (const float*)(wk + L.dx), (const float*)(wk + L.dy), v, w, u, H, W, sched.at(t),
-      dt, inv_dx2, lam_v, lam_w, lam_u, lam_v_out, lam_w_out, lam_u_out, grad_D,
-      (double*)(wk + L.part), (const AdjAxisTab*)(wk + L.tab));
-  HIP_CHECK(hipGetLastError());
+  const float* lin[3] = {lam_v, lam_w, lam_u};
+  float* lout[3] = {lam_v_out, lam_w_out, lam_u_out};
+  hipStream_t s = (hipStream_t)stream;
+  HIP_CHECK(launch_pre(L, (char*)work, D, u, H, W, lam_u, nullptr, 0, s));
+  HIP_CHECK(launch_main(L, (char*)work, c, D, v, w, u, H, W, sched.at(t), dt, inv_dx2, lin, lout,
+                        grad_D, 0, nullptr, nullptr, s));
+  return FK_OK;
+}
+
+int32_t fk_adjoint_segment(const fk_params* params, const float* D, int32_t H, int32_t W,
+                           const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                           int32_t n_steps, float dt, float dx, const float* states,
+                           const float* const* inject, float* lam_v, float* lam_w,
+                           float* lam_u, float* grad_D, void* work, size_t work_bytes,
+                           void* stream) {
+  ErrorScope es;
+  if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
+      !check_ptrs({D, lam_v, lam_w, lam_u, grad_D, work}) || !check_scalar("t", t) ||
+      !check_scalar("dt", dt) || !check_dx(dx))
+    return last_code();
+  if (n_steps < 0) return set_error(FK_ERR_ARG, "n_steps must be >= 0, got %d", n_steps);
+  if (n_steps > 0 && states == nullptr) return set_error(FK_ERR_ARG, "null states pointer");
+  if (t + (double)n_steps > 16777216.0)
+    return set_error(FK_ERR_ARG, "time index must stay below 2^24");
+  const AdjLayout L = adj_layout(H, W);
+  if (work_bytes < L.total)
+    return set_error(FK_ERR_ARG, "adjoint workspace too small: need %zu bytes, got %zu",
+                     L.total, work_bytes);
+  hipStream_t s = (hipStream_t)stream;
+  char* wk = (char*)work;
+  const size_t n = (size_t)H * W;
+  float* alt = (float*)(wk + L.lam);
+  float* bufs[2][3] = {{lam_v, lam_w, lam_u}, {alt, alt + n, alt + 2 * n}};
+  StimSchedule sched(stimuli, n_stimuli);
+  const Coef c = make_coef(*params);
+  const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
+  int cur = 0, set = 0;
+  if (n_steps > 0) {
+    const float* x = states + (size_t)(n_steps - 1) * 3 * n;
+    HIP_CHECK(launch_pre(L, wk, D, x + 2 * n, H, W, lam_u,
+                         inject ? inject[n_steps - 1] : nullptr, set, s));
+  }
+  for (int32_t k = n_steps - 1; k >= 0; --k) {
+    const float* x = states + (size_t)k * 3 * n;
+    const float* xp = k > 0 ? states + (size_t)(k - 1) * 3 * n : nullptr;
+    const float* lin[3] = {bufs[cur][0], bufs[cur][1], bufs[cur][2]};
+    HIP_CHECK(launch_main(L, wk, c, D, x, x + n, x + 2 * n, H, W, sched.at(t + (double)k), dt,
+                          inv_dx2, lin, bufs[cur ^ 1], grad_D, set, xp ? xp + 2 * n : nullptr,
+                          (xp && inject) ? inject[k - 1] : nullptr, s));
+    cur ^= 1;
+    set ^= 1;
+  }
+  if (cur == 1)
+    for (int f = 0; f < 3; ++f)
+      HIP_CHECK(hipMemcpyAsync(bufs[0][f], bufs[1][f], n * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
   return FK_OK;
 }
 
@@ -372,7 +594,7 @@ int32_t fk_adjoint_tau_grads(const void* work, int32_t H, int32_t W, float* g_ta
   if (!check_grid(H, W) || !check_ptrs({work, g_tau})) return last_code();
   const AdjLayout L = adj_layout(H, W);
   k_adj_reduce<<<1, 256, 0, (hipStream_t)stream>>>(
-      (const double*)((const char*)work + L.part), L.n_blocks, g_tau);
+      (const float*)((const char*)work + L.part), (size_t)H * W, g_tau);
   HIP_CHECK(hipGetLastError());
   return FK_OK;
 }

@@ -477,6 +477,36 @@ int32_t fk_forward_euler(const fk_params* params, const float* D, const float* v
                       t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
 }
 
+int32_t fk_forward_trajectory(const fk_params* params, const float* D, const float* v0,
+                              const float* w0, const float* u0, int32_t H, int32_t W,
+                              const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                              int32_t n_steps, float dt, float dx, float* states,
+                              void* stream) {
+  ErrorScope es;
+  if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
+      !check_ptrs({D, v0, w0, u0}) || !check_scalar("dt", dt) || !check_dx(dx))
+    return last_code();
+  if (n_steps < 0) return set_error(FK_ERR_ARG, "n_steps must be >= 0, got %d", n_steps);
+  if (n_steps == 0) return FK_OK;
+  if (states == nullptr) return set_error(FK_ERR_ARG, "null states pointer");
+  if (!check_time(t, t + (double)n_steps)) return last_code();
+  const size_t n = (size_t)H * W;
+  const Coef c = make_coef(*params);
+  const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
+  StimSchedule sched(stimuli, n_stimuli);
+  const int v = is_tb(resolve_variant(g_variant.load())) ? (int)kAuto : g_variant.load();
+  const float* src[3] = {v0, w0, u0};
+  for (int32_t k = 0; k < n_steps; ++k) {
+    float* d = states + (size_t)k * 3 * n;
+    HIP_CHECK(launch_step(v, true, c, D, src[0], src[1], src[2], H, W, sched.at(t + (double)k),
+                          dt, inv_dx2, d, d + n, d + 2 * n, (hipStream_t)stream));
+    src[0] = d;
+    src[1] = d + n;
+    src[2] = d + 2 * n;
+  }
+  return FK_OK;
+}
+
 int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const float* v_in,
                                const float* w_in, const float* u_in, int32_t H, int32_t W,
                                const fk_stimulus* stimuli, int32_t n_stimuli, double t,

@@ -114,6 +114,17 @@ int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const fl
                                float* w_out, float* u_out, void* work, size_t work_bytes,
                                void* stream);
 
+/* The states of cardiax/solve.py:92-100 after each of n_steps Euler steps
+ * from (v0, w0, u0) at time t: state k (k = 0 .. n_steps-1) is the state after
+ * step t+k, stored at states + 3*H*W*k as its v, w, u planes. Used to
+ * recompute a checkpointed segment for the adjoint. The inputs are not
+ * modified and must not alias `states`. */
+int32_t fk_forward_trajectory(const fk_params* params, const float* D, const float* v0,
+                              const float* w0, const float* u0, int32_t H, int32_t W,
+                              const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                              int32_t n_steps, float dt, float dx, float* states,
+                              void* stream);
+
 /* cardiax/solve.py:225-254 -- first derivative of a (rows, cols) array along
  * `axis` (0 or 1) without grid spacing; needs >= 5 entries along the axis. */
 int32_t fk_gradient(const float* a, float* out, int32_t rows, int32_t cols, int32_t axis,
@@ -134,7 +145,8 @@ int32_t fk_stimulate(double t, const float* X, float* out, int64_t n_cells,
  * ------------------------------------------------------------------------- */
 
 /* Scratch bytes of the adjoint for an H x W grid (tables of the boundary
- * operators, D_x, D_y and the per-block tau partial sums). */
+ * operators, D_x, D_y, the per-block tau partial sums, a cotangent
+ * ping-pong buffer and the fields derived per reverse step). */
 size_t fk_adjoint_workspace_bytes(int32_t H, int32_t W);
 
 /* Fills the workspace for D and zeroes the tau accumulators; call once per
@@ -153,6 +165,20 @@ int32_t fk_adjoint_step(const fk_params* params, const float* D, const float* v,
                         float dx, const float* lam_v, const float* lam_w, const float* lam_u,
                         float* lam_v_out, float* lam_w_out, float* lam_u_out, float* grad_D,
                         void* work, size_t work_bytes, void* stream);
+
+/* The reverse pass over n_steps Euler steps starting at time t: states holds
+ * the n_steps states x_t .. x_{t+n_steps-1} (layout of fk_forward_trajectory).
+ * On entry lam_* is the cotangent of x_{t+n_steps}; on return that of x_t.
+ * inject, if not NULL, is a host array of n_steps device pointers: a non-NULL
+ * inject[k] (H x W) is added to the u cotangent of x_{t+k+1} before step
+ * t+k is reversed (loss terms on intermediate u frames). grad_D and the tau
+ * accumulators gather every step's contribution, as fk_adjoint_step does. */
+int32_t fk_adjoint_segment(const fk_params* params, const float* D, int32_t H, int32_t W,
+                           const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                           int32_t n_steps, float dt, float dx, const float* states,
+                           const float* const* inject, float* lam_v, float* lam_w,
+                           float* lam_u, float* grad_D, void* work, size_t work_bytes,
+                           void* stream);
 
 /* Writes the accumulated gradients with respect to (tau_d, tau_r, tau_si) to
  * g_tau (3 floats, device memory). */

@@ -171,3 +171,100 @@ def test_rollout_gradient_matches_central_differences(gpu):
         fd = (L(D0 + h * dD) - L(D0 - h * dD)) / (2 * h)
         ad = float((g * dD).sum())
         assert abs(fd - ad) <= 1e-3 * abs(fd), (trial, fd, ad)
+
+
+def _dev_stims(stims, dev, shape):
+    return solve._Stimuli(stims, dev, shape)
+
+
+def test_trajectory_matches_forward_bitwise(gpu):
+    shape, n = (37, 70), 9
+    st, Dstar, stims = _c4_small(shape, 30, 10, seed=7)
+    stims = [(s[0]._replace(start=3), s[1]) for s in stims]
+    prm = P.PARAMSET_1A
+    dev = gpu
+    Lb = _lib.lib()
+    H, W = shape
+    D = torch.from_numpy(Dstar).to(dev)
+    x0 = [torch.from_numpy(a).to(dev) for a in st]
+    traj = torch.empty((n, 3, H, W), device=dev)
+    stim = _dev_stims(stims, dev, shape)
+    p = _lib.make_params(prm)
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check(Lb.fk_forward_trajectory(ctypes.byref(p), D.data_ptr(), x0[0].data_ptr(),
+                                        x0[1].data_ptr(), x0[2].data_ptr(), H, W,
+                                        stim.pointer(), stim.n, 2.0, n, 0.01, 0.01,
+                                        traj.data_ptr(), s))
+    for k in range(n):
+        ref = solve._forward_euler(st, 2, 2 + k + 1, prm, Dstar, stims, 0.01, 0.01)
+        for f in range(3):
+            assert torch.equal(traj[k, f], ref[f]), (k, f)
+
+
+def test_segment_equals_step_by_step_reversal(gpu):
+    shape, n = (29, 41), 7
+    st, Dstar, stims = _c4_small(shape, 30, 10, seed=8)
+    stims = [(s[0]._replace(start=2), s[1]) for s in stims]
+    prm = P.PARAMSET_3
+    dev = gpu
+    Lb = _lib.lib()
+    H, W = shape
+    D = torch.from_numpy(Dstar).to(dev)
+    x0 = [torch.from_numpy(a).to(dev) for a in st]
+    states = torch.empty((n, 3, H, W), device=dev)
+    for f in range(3):
+        states[0, f].copy_(x0[f])
+    stim = _dev_stims(stims, dev, shape)
+    p = _lib.make_params(prm)
+    s = torch.cuda.current_stream().cuda_stream
+    t0 = 1.0
+    _lib.check(Lb.fk_forward_trajectory(ctypes.byref(p), D.data_ptr(), x0[0].data_ptr(),
+                                        x0[1].data_ptr(), x0[2].data_ptr(), H, W,
+                                        stim.pointer(), stim.n, t0, n - 1, 0.01, 0.01,
+                                        states[1].data_ptr(), s))
+    g = torch.Generator(device="cpu").manual_seed(3)
+    lam0 = [torch.randn(shape, generator=g).to(dev) for _ in range(3)]
+    inj = {2: torch.randn(shape, generator=g).to(dev), 5: torch.randn(shape, generator=g).to(dev)}
+    nb = Lb.fk_adjoint_workspace_bytes(H, W)
+
+    def fresh():
+        work = torch.empty(nb // 8 + 1, dtype=torch.float64, device=dev)
+        _lib.check(Lb.fk_adjoint_prepare(D.data_ptr(), H, W, work.data_ptr(), nb, s))
+        return work
+
+    # one segment call
+    work = fresh()
+    lam = [x.clone() for x in lam0]
+    gD = torch.zeros_like(D)
+    ptrs = (ctypes.c_void_p * n)()
+    for k, x in inj.items():
+        ptrs[k] = x.data_ptr()
+    _lib.check(Lb.fk_adjoint_segment(ctypes.byref(p), D.data_ptr(), H, W, stim.pointer(),
+                                     stim.n, t0, n, 0.01, 0.01, states.data_ptr(), ptrs,
+                                     lam[0].data_ptr(), lam[1].data_ptr(), lam[2].data_ptr(),
+                                     gD.data_ptr(), work.data_ptr(), nb, s))
+    gt = torch.empty(3, device=dev)
+    _lib.check(Lb.fk_adjoint_tau_grads(work.data_ptr(), H, W, gt.data_ptr(), s))
+    # step by step
+    work2 = fresh()
+    cur = [x.clone() for x in lam0]
+    gD2 = torch.zeros_like(D)
+    for k in range(n - 1, -1, -1):
+        if k in inj:
+            cur[2] = cur[2] + inj[k]
+        out = [torch.empty_like(x) for x in cur]
+        st_k = states[k]
+        _lib.check(Lb.fk_adjoint_step(ctypes.byref(p), D.data_ptr(), st_k[0].data_ptr(),
+                                      st_k[1].data_ptr(), st_k[2].data_ptr(), H, W,
+                                      stim.pointer(), stim.n, t0 + k, 0.01, 0.01,
+                                      cur[0].data_ptr(), cur[1].data_ptr(), cur[2].data_ptr(),
+                                      out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(),
+                                      gD2.data_ptr(), work2.data_ptr(), nb, s))
+        cur = out
+    gt2 = torch.empty(3, device=dev)
+    _lib.check(Lb.fk_adjoint_tau_grads(work2.data_ptr(), H, W, gt2.data_ptr(), s))
+    torch.cuda.synchronize()
+    for a, b in zip(lam, cur):
+        assert torch.equal(a, b)
+    assert torch.equal(gD, gD2)
+    assert torch.equal(gt, gt2)
