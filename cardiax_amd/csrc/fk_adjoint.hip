@@ -174,10 +174,13 @@ __device__ __forceinline__ bool central_cone(int j, int N) {
 
 // Fields the reverse step derives from u and the incoming cotangent before
 // applying the transposed operators: L = lam_u (+ injected cotangent),
-// Q2 = D L, Q1x = D_x L, Q1y = D_y L, Px = L (A1x u), Py = L (A1y u), and
-// lap = A2x u + A2y u.
+// lap = A2x u + A2y u, Q2 = D L, Q1x = D_x L, Px = L (A1x u), Q1y = D_y L,
+// Py = L (A1y u). Fields read together are stored as pairs.
 struct AdjPre {
-  float *L, *Q2, *Q1x, *Q1y, *Px, *Py, *lap;
+  float2* Llap;  // (L, lap)
+  float2* QPx;   // (Q1x, Px)
+  float2* QPy;   // (Q1y, Py)
+  float* Q2;
 };
 
 // Inputs of the derived fields of one cell: u at offsets -5..5 along both
@@ -242,13 +245,10 @@ __device__ __forceinline__ void pre_store(const PreIn& in, float L, int i, int c
       lap += on ? a2 * in.ys[d + kAdjBand] : 0.0f;
     }
   }
-  o.L[idx] = L;
+  o.Llap[idx] = make_float2(L, lap);
+  o.QPx[idx] = make_float2(in.Dx * L, L * gx);
+  o.QPy[idx] = make_float2(in.Dy * L, L * gy);
   o.Q2[idx] = in.D * L;
-  o.Q1x[idx] = in.Dx * L;
-  o.Q1y[idx] = in.Dy * L;
-  o.Px[idx] = L * gx;
-  o.Py[idx] = L * gy;
-  o.lap[idx] = lap;
 }
 
 __global__ void __launch_bounds__(kAdjThreads)
@@ -276,7 +276,7 @@ struct AdjNext {
 
 // One reverse step from the derived fields. Each cell adds its contribution
 // to the tau gradients to its own entry of the three accumulator fields
-// g_tau[k * H * W + cell] (deterministic; summed by k_adj_reduce).
+// g_tau[cell].{x, y, z} (deterministic; summed by k_adj_reduce).
 // NEXT: also write the derived fields of the previous step's reverse (they
 // depend on the new cotangent only at the same cell), so a segment needs one
 // launch per step.
@@ -286,13 +286,12 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
            const float* __restrict__ u, int H, int W, StimArgs sa, float dt, float inv_dx2,
            const float* __restrict__ lv, const float* __restrict__ lw, AdjPre pre,
            float* __restrict__ lv_o, float* __restrict__ lw_o, float* __restrict__ lu_o,
-           float* __restrict__ gD, float* __restrict__ g_tau,
+           float* __restrict__ gD, float4* __restrict__ g_tau,
            const AdjAxisTab* __restrict__ tab, AdjNext nx) {
   const int col = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i < H && col < W) {
     const size_t idx = (size_t)i * W + col;
-    const size_t n = (size_t)H * W;
     PreIn next{};
     float inj = 0.0f;
     if (NEXT) {  // issued first: independent of everything below
@@ -300,7 +299,8 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
       if (nx.inj) inj = nx.inj[idx];
     }
     const float vv = v[idx], ww = w[idx], uu = u[idx];
-    const float lvv = lv[idx], lww = lw[idx], L = pre.L[idx];
+    const float2 Llap = pre.Llap[idx];
+    const float lvv = lv[idx], lww = lw[idx], L = Llap.x;
 
     // --- reaction, gates as fixed masks ---------------------------------
     const bool p = uu >= c.V_c;
@@ -331,8 +331,9 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
         const size_t j = (size_t)(i - d) * W + col;
         sx += FK_A2(d) * pre.Q2[j];
         if (d >= -2 && d <= 2 && d != 0) {
-          sx += FK_A1(d) * pre.Q1x[j];
-          ex += FK_A1(d) * pre.Px[j];
+          const float2 q = pre.QPx[j];
+          sx += FK_A1(d) * q.x;
+          ex += FK_A1(d) * q.y;
         }
       }
     } else {
@@ -342,8 +343,9 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
         float a1, a2;
         band_coef(tab[0], jr, d, H, a1, a2);  // 0 when jr is off the axis
         const size_t j = (size_t)clampi(jr, 0, H - 1) * W + col;
-        sx += a2 * pre.Q2[j] + a1 * pre.Q1x[j];
-        ex += a1 * pre.Px[j];
+        const float2 q = pre.QPx[j];
+        sx += a2 * pre.Q2[j] + a1 * q.x;
+        ex += a1 * q.y;
       }
     }
     if (central_cone(col, W)) {
@@ -352,8 +354,9 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
         const size_t j = idx - d;
         sy += FK_A2(d) * pre.Q2[j];
         if (d >= -2 && d <= 2 && d != 0) {
-          sy += FK_A1(d) * pre.Q1y[j];
-          ey += FK_A1(d) * pre.Py[j];
+          const float2 q = pre.QPy[j];
+          sy += FK_A1(d) * q.x;
+          ey += FK_A1(d) * q.y;
         }
       }
     } else {
@@ -363,8 +366,9 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
         float a1, a2;
         band_coef(tab[1], jc, d, W, a1, a2);  // 0 when jc is off the axis
         const size_t j = (size_t)i * W + clampi(jc, 0, W - 1);
-        sy += a2 * pre.Q2[j] + a1 * pre.Q1y[j];
-        ey += a1 * pre.Py[j];
+        const float2 q = pre.QPy[j];
+        sy += a2 * pre.Q2[j] + a1 * q.x;
+        ey += a1 * q.y;
       }
     }
 
@@ -373,16 +377,18 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
     const float lu_new = L + dt * (djdu * lus + (sx + sy) * inv_dx2);
     lu_o[idx] = lu_new;
     if (NEXT) pre_store(next, lu_new + inj, i, col, H, W, tab, nx.pre);
-    gD[idx] += dt * (L * pre.lap[idx] + ex + ey) * inv_dx2;
+    gD[idx] += dt * (L * Llap.y + ex + ey) * inv_dx2;
     // tau gradients: d j_ion / d tau
     const float djd_td = p ? -vv * (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_td * c.inv_cm
                            : 0.0f;
     const float djd_tr = p ? c.inv_tr * c.inv_tr * c.inv_cm : 0.0f;
     const float djd_tsi = -ww * S * c.inv_tsi * c.inv_tsi * c.inv_cm;
     // per-cell accumulators of the tau gradients (reduced once at the end)
-    g_tau[idx] += dt * djd_td * lus;
-    g_tau[n + idx] += dt * djd_tr * lus;
-    g_tau[2 * n + idx] += dt * djd_tsi * lus;
+    float4 acc = g_tau[idx];
+    acc.x += dt * djd_td * lus;
+    acc.y += dt * djd_tr * lus;
+    acc.z += dt * djd_tsi * lus;
+    g_tau[idx] = acc;
   }
 }
 
@@ -392,15 +398,20 @@ __global__ void k_adj_zero(float* a, size_t n) {
     a[i] = 0.0f;
 }
 
-// Sum of each of the three per-cell tau accumulators (n cells each), in
+// Sums of the three per-cell tau accumulators (x, y, z of n float4), in
 // double.
-__global__ void k_adj_reduce(const float* __restrict__ part, size_t n, float* g_tau) {
+__global__ void k_adj_reduce(const float4* __restrict__ part, size_t n, float* g_tau) {
   __shared__ double s[3][256];
-  for (int k = 0; k < 3; ++k) {
-    double acc = 0.0;
-    for (size_t b = threadIdx.x; b < n; b += blockDim.x) acc += (double)part[k * n + b];
-    s[k][threadIdx.x] = acc;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (size_t b = threadIdx.x; b < n; b += blockDim.x) {
+    const float4 p = part[b];
+    a0 += (double)p.x;
+    a1 += (double)p.y;
+    a2 += (double)p.z;
   }
+  s[0][threadIdx.x] = a0;
+  s[1][threadIdx.x] = a1;
+  s[2][threadIdx.x] = a2;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o)
@@ -411,8 +422,8 @@ __global__ void k_adj_reduce(const float* __restrict__ part, size_t n, float* g_
     for (int k = 0; k < 3; ++k) g_tau[k] = (float)s[k][0];
 }
 
-// Workspace layout: [tables (2 axes)][Dx][Dy][tau accumulators: 3 fields][cotangent
-// ping-pong: 3 fields][derived fields: 2 sets of 7]
+// Workspace layout: [tables (2 axes)][Dx][Dy][tau accumulators: float4 per cell][cotangent
+// ping-pong: 3 fields][derived fields: 2 sets of 7 (8 floats per cell)]
 struct AdjLayout {
   size_t tab, dx, dy, part, lam, pre, total;
 };
@@ -429,16 +440,18 @@ static AdjLayout adj_layout(int H, int W) {
   L.tab = take(2 * sizeof(AdjAxisTab));
   L.dx = take(cells * sizeof(float));
   L.dy = take(cells * sizeof(float));
-  L.part = take(3 * cells * sizeof(float));
+  L.part = take(cells * sizeof(float4));
   L.lam = take(3 * cells * sizeof(float));
-  L.pre = take(2 * 7 * cells * sizeof(float));
+  L.pre = take(2 * 8 * cells * sizeof(float));
   L.total = o;
   return L;
 }
 
+// Set `set` of the derived fields: three float2 fields, then Q2 (the pairs
+// stay 8-byte aligned for any n).
 static AdjPre pre_fields(const AdjLayout& L, char* wk, int set, size_t n) {
-  float* pf = (float*)(wk + L.pre) + (size_t)set * 7 * n;
-  return AdjPre{pf, pf + n, pf + 2 * n, pf + 3 * n, pf + 4 * n, pf + 5 * n, pf + 6 * n};
+  float* pf = (float*)(wk + L.pre) + (size_t)set * 8 * n;
+  return AdjPre{(float2*)pf, (float2*)(pf + 2 * n), (float2*)(pf + 4 * n), pf + 6 * n};
 }
 
 static hipError_t launch_pre(const AdjLayout& L, char* wk, const float* D, const float* u,
@@ -469,11 +482,11 @@ static hipError_t launch_main(const AdjLayout& L, char* wk, const Coef& c, const
   if (u_prev)
     k_adj_main<true><<<grid, kAdjThreads, 0, s>>>(
         c, v, w, u, H, W, sa, dt, inv_dx2, lam_in[0], lam_in[1], pre_fields(L, wk, set, n),
-        lam_out[0], lam_out[1], lam_out[2], gD, (float*)(wk + L.part), tab, nx);
+        lam_out[0], lam_out[1], lam_out[2], gD, (float4*)(wk + L.part), tab, nx);
   else
     k_adj_main<false><<<grid, kAdjThreads, 0, s>>>(
         c, v, w, u, H, W, sa, dt, inv_dx2, lam_in[0], lam_in[1], pre_fields(L, wk, set, n),
-        lam_out[0], lam_out[1], lam_out[2], gD, (float*)(wk + L.part), tab, nx);
+        lam_out[0], lam_out[1], lam_out[2], gD, (float4*)(wk + L.part), tab, nx);
   return hipGetLastError();
 }
 
@@ -505,7 +518,7 @@ int32_t fk_adjoint_prepare(const float* D, int32_t H, int32_t W, void* work, siz
   const dim3 block(64, 4), grid((W + 63) / 64, (H + 3) / 4);
   k_adj_dgrad<<<grid, block, 0, s>>>(D, H, W, tab, (float*)(wk + L.dx), (float*)(wk + L.dy));
   HIP_CHECK(hipGetLastError());
-  k_adj_zero<<<256, 256, 0, s>>>((float*)(wk + L.part), 3 * (size_t)H * W);
+  k_adj_zero<<<256, 256, 0, s>>>((float*)(wk + L.part), 4 * (size_t)H * W);
   HIP_CHECK(hipGetLastError());
   return FK_OK;
 }
@@ -594,7 +607,7 @@ int32_t fk_adjoint_tau_grads(const void* work, int32_t H, int32_t W, float* g_ta
   if (!check_grid(H, W) || !check_ptrs({work, g_tau})) return last_code();
   const AdjLayout L = adj_layout(H, W);
   k_adj_reduce<<<1, 256, 0, (hipStream_t)stream>>>(
-      (const float*)((const char*)work + L.part), (size_t)H * W, g_tau);
+      (const float4*)((const char*)work + L.part), (size_t)H * W, g_tau);
   HIP_CHECK(hipGetLastError());
   return FK_OK;
 }
