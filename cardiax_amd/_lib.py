@@ -61,6 +61,10 @@ _SIGNATURES = {
     "fk_last_error": (ctypes.c_char_p, []),
     "fk_abi_version": (c_i32, []),
     "fk_set_forward_variant": (c_i32, [c_i32]),
+    "fk_autotune_forward": (c_i32, [c_i32, c_i32, c_i32, c_vp, ctypes.POINTER(c_f),
+                                    ctypes.POINTER(c_i32)]),
+    "fk_autotune_lookup": (c_i32, [c_i32, c_i32]),
+    "fk_autotune_reset": (None, []),
     "fk_init": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "fk_step": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
                         ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_vp, c_vp, c_vp, c_vp]),
@@ -91,6 +95,9 @@ _SIGNATURES = {
     "fk_adjoint_tau_grads": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_vp]),
     "fk_batch_create": (c_i32, [ctypes.POINTER(FkSim), c_i32, ctypes.POINTER(c_vp)]),
     "fk_batch_run": (c_i32, [c_vp, c_i64, c_i32, c_vp]),
+    "fk_batch_autotune": (c_i32, [c_vp, c_i32, c_vp, ctypes.POINTER(c_f),
+                                  ctypes.POINTER(c_i32)]),
+    "fk_batch_set_rows": (c_i32, [c_vp, c_i32]),
     "fk_batch_n_items": (c_i32, [c_vp]),
     "fk_batch_destroy": (c_i32, [c_vp]),
 }

@@ -94,6 +94,24 @@ class Batch:
     def n_tiles(self) -> int:
         return int(_lib.lib().fk_batch_n_items(self._h))
 
+    def autotune(self, n_steps: int = 64) -> dict:
+        """Time the batched step with 8- and 16-row bands and keep the faster
+        for later runs (fk_batch_autotune); the simulations' states are not
+        touched. Returns ``{"rows": R, "times_ms": [t8, t16]}``."""
+        if self._h is None:
+            raise RuntimeError("batch is closed")
+        times = (ctypes.c_float * 2)()
+        rows = ctypes.c_int32(0)
+        _lib.check(_lib.lib().fk_batch_autotune(self._h, int(n_steps), stream_handle(), times,
+                                            ctypes.byref(rows)))
+        return {"rows": int(rows.value), "times_ms": [round(float(t), 4) for t in times]}
+
+    def set_rows(self, rows: int) -> None:
+        """Pin the band height (8 or 16) of later runs (fk_batch_set_rows)."""
+        if self._h is None:
+            raise RuntimeError("batch is closed")
+        _lib.check(_lib.lib().fk_batch_set_rows(self._h, int(rows)))
+
     def run(self, n_steps: int, use_graph: bool = True) -> List[State]:
         """Advance all simulations by ``n_steps`` Euler steps (asynchronous on
         torch's current stream)."""

@@ -27,7 +27,8 @@
 
 namespace fk {
 
-constexpr int kBatchR = 8;        // rows per wave band
+// Rows per wave band: one work list per option, chosen by fk_batch_autotune.
+constexpr int kBatchRows[2] = {8, 16};
 constexpr int kGraphSteps = 32;   // Euler steps recorded in one graph (even)
 
 struct BatchStim {
@@ -95,7 +96,9 @@ struct BatchStimView {
   __device__ const float* field(int m) const { return s[m].field; }
 };
 
-// One Euler step of every tile. Reads state[parity], writes state[parity ^ 1].
+// One Euler step of every tile (bands of R rows). Reads state[parity],
+// writes state[parity ^ 1].
+template <int R>
 __global__ void __launch_bounds__(256)
 k_batch_step(const SimDesc* __restrict__ sims, const TileItem* __restrict__ items, int n_items,
              const uint32_t* __restrict__ masks, int parity) {
@@ -127,7 +130,7 @@ k_batch_step(const SimDesc* __restrict__ sims, const TileItem* __restrict__ item
   const float dt = uni(S.dt), inv_dx2 = uni(S.inv_dx2);
   const int sel = item.flags | (mask ? 4 : 0);
 #define FK_BT(X, Y, ST)                                                                    \
-  stream_tile<kBatchR, true, X, Y, ST>(c, Dp, i0, i1, i2, H, W, sv, dt, inv_dx2, o0, o1, o2, \
+  stream_tile<R, true, X, Y, ST>(c, Dp, i0, i1, i2, H, W, sv, dt, inv_dx2, o0, o1, o2,       \
                                        item.rs, item.cs)
   switch (sel) {
     case 0: FK_BT(false, false, false); break;
@@ -163,13 +166,14 @@ using namespace fk;
 // Opaque batch handle.
 struct fk_batch {
   int n_sims = 0;
-  int n_items = 0;
+  int opt = 0;            // index into kBatchRows
+  int n_items_opt[2] = {0, 0};
   std::vector<fk_sim> host_sims;   // caller's descriptors (state pointers)
   std::vector<size_t> offset;      // per-sim cell offset in the pools
   size_t total_cells = 0;
   char* dev = nullptr;             // one allocation: descriptors, items, counter, pools
   SimDesc* d_sims = nullptr;
-  TileItem* d_items = nullptr;
+  TileItem* d_items[2] = {nullptr, nullptr};
   int64_t* d_step = nullptr;
   uint32_t* d_mask = nullptr;  // per-simulation stimulus activity of the current step
   float* pool[2] = {nullptr, nullptr};  // ping-pong pools of 3 * total_cells floats
@@ -213,28 +217,35 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out) {
   fk_batch* b = new fk_batch();
   b->n_sims = n;
   b->host_sims.assign(sims, sims + n);
-  std::vector<TileItem> items;
+  std::vector<TileItem> items[2];
   b->offset.resize(n);
   for (int i = 0; i < n; ++i) {
     const fk_sim& s = sims[i];
     b->offset[i] = b->total_cells;
     b->total_cells += (size_t)s.H * s.W;
-    const StreamGeom g = make_stream_geom<kBatchR>(s.H, s.W);
-    for (int strip = 0; strip < g.n_strips; ++strip)
-      for (int band = 0; band < g.n_bands; ++band) {
-        TileItem t;
-        t.sim = i;
-        t.cs = std::min(kStripOut * strip, g.cs_last);
-        t.rs = std::min(kBatchR * band, g.rs_last);
-        const bool xg = t.rs < 3 || t.rs + kBatchR + 3 > s.H;
-        const bool yg = t.cs < 3 || t.cs + kStripOut + 3 > s.W;
-        t.flags = (xg ? 1 : 0) | (yg ? 2 : 0);
-        items.push_back(t);
-      }
+    for (int o = 0; o < 2; ++o) {
+      const int R = kBatchRows[o];
+      const StreamGeom g = o == 0 ? make_stream_geom<kBatchRows[0]>(s.H, s.W)
+                                  : make_stream_geom<kBatchRows[1]>(s.H, s.W);
+      for (int strip = 0; strip < g.n_strips; ++strip)
+        for (int band = 0; band < g.n_bands; ++band) {
+          TileItem t;
+          t.sim = i;
+          t.cs = std::min(kStripOut * strip, g.cs_last);
+          t.rs = std::min(R * band, g.rs_last);
+          const bool xg = t.rs < 3 || t.rs + R + 3 > s.H;
+          const bool yg = t.cs < 3 || t.cs + kStripOut + 3 > s.W;
+          t.flags = (xg ? 1 : 0) | (yg ? 2 : 0);
+          items[o].push_back(t);
+        }
+    }
   }
-  b->n_items = (int)items.size();
+  b->n_items_opt[0] = (int)items[0].size();
+  b->n_items_opt[1] = (int)items[1].size();
   const size_t sz_sims = ((sizeof(SimDesc) * n + 255) / 256) * 256;
-  const size_t sz_items = ((sizeof(TileItem) * items.size() + 255) / 256) * 256;
+  const size_t sz_it0 = ((sizeof(TileItem) * items[0].size() + 255) / 256) * 256;
+  const size_t sz_it1 = ((sizeof(TileItem) * items[1].size() + 255) / 256) * 256;
+  const size_t sz_items = sz_it0 + sz_it1;
   const size_t sz_pool = 3 * b->total_cells * sizeof(float);
   const size_t sz_mask = ((sizeof(uint32_t) * n + 255) / 256) * 256;
   const size_t total = sz_sims + sz_items + 256 + sz_mask + 2 * sz_pool;
@@ -244,7 +255,8 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out) {
     return set_error(FK_ERR_HIP, "hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
   }
   b->d_sims = (SimDesc*)b->dev;
-  b->d_items = (TileItem*)(b->dev + sz_sims);
+  b->d_items[0] = (TileItem*)(b->dev + sz_sims);
+  b->d_items[1] = (TileItem*)(b->dev + sz_sims + sz_it0);
   b->d_step = (int64_t*)(b->dev + sz_sims + sz_items);
   b->d_mask = (uint32_t*)(b->dev + sz_sims + sz_items + 256);
   b->pool[0] = (float*)(b->dev + sz_sims + sz_items + 256 + sz_mask);
@@ -270,8 +282,8 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out) {
                    s.stimuli[k].field};
   }
   e = hipMemcpy(b->d_sims, desc.data(), sizeof(SimDesc) * n, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(b->d_items, items.data(), sizeof(TileItem) * items.size(),
+  for (int o = 0; o < 2 && e == hipSuccess; ++o)
+    e = hipMemcpy(b->d_items[o], items[o].data(), sizeof(TileItem) * items[o].size(),
                   hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -290,13 +302,19 @@ int32_t fk_batch_destroy(fk_batch* b) {
   return FK_OK;
 }
 
-int32_t fk_batch_n_items(const fk_batch* b) { return b ? b->n_items : 0; }
+int32_t fk_batch_n_items(const fk_batch* b) { return b ? b->n_items_opt[b->opt] : 0; }
 
 static hipError_t batch_step(fk_batch* b, int rel, int parity, hipStream_t s) {
   k_batch_activity<<<(b->n_sims + 63) / 64, 64, 0, s>>>(b->d_sims, b->n_sims, b->d_step, rel,
                                                         b->d_mask);
-  const int blocks = (b->n_items + 3) / 4;
-  k_batch_step<<<blocks, 256, 0, s>>>(b->d_sims, b->d_items, b->n_items, b->d_mask, parity);
+  const int n_items = b->n_items_opt[b->opt];
+  const int blocks = (n_items + 3) / 4;
+  if (b->opt == 0)
+    k_batch_step<kBatchRows[0]><<<blocks, 256, 0, s>>>(b->d_sims, b->d_items[0], n_items,
+                                                       b->d_mask, parity);
+  else
+    k_batch_step<kBatchRows[1]><<<blocks, 256, 0, s>>>(b->d_sims, b->d_items[1], n_items,
+                                                       b->d_mask, parity);
   return hipGetLastError();
 }
 
@@ -316,6 +334,69 @@ static hipError_t batch_build_graph(fk_batch* b, int parity) {
   if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[parity], g, nullptr, nullptr, 0);
   if (g) (void)hipGraphDestroy(g);
   return e;
+}
+
+int32_t fk_batch_autotune(fk_batch* b, int32_t n_steps, void* stream, float* times_ms,
+                          int32_t* best_rows) {
+  ErrorScope es;
+  if (b == nullptr) return set_error(FK_ERR_ARG, "batch is null");
+  if (n_steps < 1 || n_steps > 10000)
+    return set_error(FK_ERR_ARG, "n_steps must be in [1, 10000], got %d", n_steps);
+  hipStream_t s = (hipStream_t)stream;
+  // Queued runs may still replay this batch's graphs or use its pools.
+  HIP_CHECK(hipDeviceSynchronize());
+  // The pools are scratch between runs (fk_batch_run copies the state in),
+  // so timing may overwrite them; the step counter is reset by every run.
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float ms[2] = {0.0f, 0.0f};
+  for (int o = 0; o < 2 && e == hipSuccess; ++o) {
+    b->opt = o;
+    k_batch_set<<<1, 64, 0, s>>>(b->d_step, 0);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = batch_step(b, k, k & 1, s);
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int k = 0; k < n_steps && e == hipSuccess; ++k) e = batch_step(b, k, k & 1, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[o], e0, e1);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  b->opt = (e == hipSuccess && ms[1] < ms[0]) ? 1 : 0;
+  // graphs recorded for another work list are stale
+  for (auto& x : b->exec)
+    if (x) {
+      (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    }
+  if (e != hipSuccess) return set_error(FK_ERR_HIP, "batch autotune failed: %s",
+                                        hipGetErrorString(e));
+  if (times_ms) {
+    times_ms[0] = ms[0];
+    times_ms[1] = ms[1];
+  }
+  if (best_rows) *best_rows = kBatchRows[b->opt];
+  return FK_OK;
+}
+
+int32_t fk_batch_set_rows(fk_batch* b, int32_t rows) {
+  ErrorScope es;
+  if (b == nullptr) return set_error(FK_ERR_ARG, "batch is null");
+  int o = -1;
+  for (int k = 0; k < 2; ++k)
+    if (rows == kBatchRows[k]) o = k;
+  if (o < 0) return set_error(FK_ERR_ARG, "band height must be 8 or 16, got %d", rows);
+  if (o != b->opt) {
+    HIP_CHECK(hipDeviceSynchronize());  // no queued launch may still use the old list
+    for (auto& x : b->exec)
+      if (x) {
+        (void)hipGraphExecDestroy(x);
+        x = nullptr;
+      }
+    b->opt = o;
+  }
+  return FK_OK;
 }
 
 int32_t fk_batch_run(fk_batch* b, int64_t n_steps, int32_t use_graph, void* stream) {

@@ -12,9 +12,13 @@
 //            HBM pass per step: read v, w, u, D, write v, w, u.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "cardiax_amd.h"
@@ -202,6 +206,12 @@ k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
 // ---------------------------------------------------------------------------
 // Small kernels: init, gradient, stimulate
 // ---------------------------------------------------------------------------
+__global__ void k_fill(float* a, size_t n, float x) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    a[i] = x;
+}
+
 __global__ void k_init(float* v, float* w, float* u, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -262,9 +272,29 @@ static bool is_tb(int v) { return v >= kTb64x2 && v <= kTb32x4; }
 
 static std::atomic<int> g_variant{kAuto};
 
-// Default when no variant is set: the streaming kernel with 8-row bands, the
-// fastest single-step kernel measured on MI355X from 128^2 to 2048^2.
+// Autotuner results: grid shape -> fastest variant measured on this device
+// (fk_autotune_forward). Read on every automatic launch, written by the
+// tuner; guarded by a mutex.
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, int> g_tuned;
+
+static int tuned_variant(int H, int W) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const auto it = g_tuned.find({H, W});
+  return it == g_tuned.end() ? (int)kAuto : it->second;
+}
+
+// Default when no variant is set and the shape was not tuned: the streaming
+// kernel with 8-row bands, the fastest single-step kernel measured on MI355X
+// from 128^2 to 2048^2.
 static int resolve_variant(int v) { return v == kAuto ? (int)kStream8 : v; }
+
+// The variant an automatic launch on an H x W grid uses.
+static int resolve_for(int v, int H, int W) {
+  if (v != kAuto) return v;
+  const int t = tuned_variant(H, W);
+  return t != kAuto ? t : (int)kStream8;
+}
 
 template <int R>
 static void launch_stream(bool euler, const Coef& c, const float* D, const float* v,
@@ -374,7 +404,7 @@ static int32_t forward_loop(int variant, const fk_params* params, const float* D
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
   float* wk = (float*)work;
   float* bufs[2][3] = {{v_out, w_out, u_out}, {wk, wk + n, wk + 2 * n}};
-  const int v = resolve_variant(variant);
+  const int v = resolve_for(variant, H, W);
   const int depth = is_tb(v) ? tb_depth(v) : 1;
   const int64_t launches = (steps + depth - 1) / depth;
   // Ping-pong so that the last launch lands in the output buffers.
@@ -441,8 +471,8 @@ int32_t fk_step(const fk_params* params, const float* D, const float* v, const f
   StimSchedule sched(stimuli, n_stimuli);
   const Coef c = make_coef(*params);
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
-  HIP_CHECK(launch_step(g_variant.load(), false, c, D, v, w, u, H, W, sched.at(t), 0.0f, inv_dx2,
-                        dv, dw, du, (hipStream_t)stream));
+  HIP_CHECK(launch_step(resolve_for(g_variant.load(), H, W), false, c, D, v, w, u, H, W,
+                        sched.at(t), 0.0f, inv_dx2, dv, dw, du, (hipStream_t)stream));
   return FK_OK;
 }
 
@@ -458,8 +488,8 @@ int32_t fk_step_euler(const fk_params* params, const float* D, const float* v,
   StimSchedule sched(stimuli, n_stimuli);
   const Coef c = make_coef(*params);
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
-  HIP_CHECK(launch_step(g_variant.load(), true, c, D, v, w, u, H, W, sched.at(t), dt, inv_dx2,
-                        v_out, w_out, u_out, (hipStream_t)stream));
+  HIP_CHECK(launch_step(resolve_for(g_variant.load(), H, W), true, c, D, v, w, u, H, W,
+                        sched.at(t), dt, inv_dx2, v_out, w_out, u_out, (hipStream_t)stream));
   return FK_OK;
 }
 
@@ -475,6 +505,75 @@ int32_t fk_forward_euler(const fk_params* params, const float* D, const float* v
                          float* u_out, void* work, size_t work_bytes, void* stream) {
   return forward_loop(g_variant.load(), params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli, t,
                       t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
+}
+
+int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
+                            float* times_ms, int32_t* best) {
+  ErrorScope es;
+  if (!check_grid(H, W)) return last_code();
+  if (n_steps < 1 || n_steps > 100000)
+    return set_error(FK_ERR_ARG, "n_steps must be in [1, 100000], got %d", n_steps);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)H * W;
+  float* mem = nullptr;
+  HIP_CHECK(hipMalloc((void**)&mem, 10 * n * sizeof(float)));
+  float *D = mem, *x = mem + n, *y = mem + 4 * n, *wk = mem + 7 * n;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  k_fill<<<blocks, 256, 0, s>>>(D, n, 1e-3f);
+  k_init<<<blocks, 256, 0, s>>>(x, x + n, x + 2 * n, (int64_t)n);
+  // PARAMSET_3 (cardiax/params.py:60-62); the kernels' cost does not depend on values
+  const fk_params p{3.33f, 19.6f, 1250.0f, 870.0f, 41.0f, 0.25f, 12.5f,
+                    33.33f, 29.0f, 10.0f, 0.85f, 0.13f, 0.04f, 1.0f};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int best_v = kStream8;
+  float best_ms = 0.0f;
+  const int cand[] = {kStream8, kStream16, kStream32, kTile16, kTile32,
+                      kTb64x2,  kTb64x4,   kTb32x2,   kTb32x4};
+  if (times_ms)
+    for (int k = 0; k < kNumVariants; ++k) times_ms[k] = -1.0f;
+  const size_t wb = 3 * n * sizeof(float);
+  for (int v : cand) {
+    if (e != hipSuccess) break;
+    // warm-up launch, then the timed run
+    int32_t rc = forward_loop(v, &p, D, x, x + n, x + 2 * n, H, W, nullptr, 0, 0.0, 8.0, 0.01f,
+                              0.01f, y, y + n, y + 2 * n, wk, wb, s);
+    if (rc == FK_OK) e = hipEventRecord(e0, s);
+    if (rc == FK_OK && e == hipSuccess)
+      rc = forward_loop(v, &p, D, x, x + n, x + 2 * n, H, W, nullptr, 0, 0.0, (double)n_steps,
+                        0.01f, 0.01f, y, y + n, y + 2 * n, wk, wb, s);
+    if (rc == FK_OK && e == hipSuccess) e = hipEventRecord(e1, s);
+    if (rc == FK_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (rc == FK_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (rc != FK_OK || e != hipSuccess) break;
+    if (times_ms) times_ms[v] = ms;
+    if (v == cand[0] || ms < best_ms) {
+      best_ms = ms;
+      best_v = v;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  const hipError_t ef = hipFree(mem);
+  if (e == hipSuccess) e = ef;
+  if (last_code() != FK_OK) return last_code();
+  if (e != hipSuccess)
+    return set_error(FK_ERR_HIP, "autotune failed: %s", hipGetErrorString(e));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[{H, W}] = best_v;
+  }
+  if (best) *best = best_v;
+  return FK_OK;
+}
+
+int32_t fk_autotune_lookup(int32_t H, int32_t W) { return tuned_variant(H, W); }
+
+void fk_autotune_reset(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned.clear();
 }
 
 int32_t fk_forward_trajectory(const fk_params* params, const float* D, const float* v0,
@@ -494,7 +593,7 @@ int32_t fk_forward_trajectory(const fk_params* params, const float* D, const flo
   const Coef c = make_coef(*params);
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
   StimSchedule sched(stimuli, n_stimuli);
-  const int v = is_tb(resolve_variant(g_variant.load())) ? (int)kAuto : g_variant.load();
+  const int v = resolve_for(g_variant.load(), H, W);  // temporal-blocking variants: stream8
   const float* src[3] = {v0, w0, u0};
   for (int32_t k = 0; k < n_steps; ++k) {
     float* d = states + (size_t)k * 3 * n;

@@ -61,21 +61,21 @@ def rect_mask(shape, centre, size, modulus):
 
 
 def scar_map(shape, rng, n_min=1, n_max=5, axes=(20, 200), scale=(0.1, 0.2), base=D0,
-             taper=(3, 5), taper_frac=0.1):
+             taper=(3, 5), taper_frac=0.1, min_taper=12.0):
     """Baseline diffusivity with elliptical scars.
 
     Each scar: uniform centre, full axes U[axes] cells (scaled by side/1024 for
     grids other than 1024), rotation U[0, pi), D multiplied inside by
-    U[scale], with a smoothstep edge max(U[taper], taper_frac * side) cells
-    wide. The edge scales with the grid like the reference's blur, whose
-    kernel spans 10% of the image (deepx/utils_scars.py:402-426), and the
-    scale stays >= 0.1 (overlaps take the minimum, not the product), the floor
-    of the reference's own random D range
-    [1e-4, 1e-3] (deepx/generate.py:80). Both keep the dynamics bounded: the
-    reference's non-conservative D (u_xx + u_yy) + D_x u_x + D_y u_y with its
-    stencils has growing modes where D changes steeply (edges of 3% of the
-    side, or scales near 0, diverge on 64^2 grids for most parameter sets),
-    which excited tissue does not damp.
+    U[scale], with a smoothstep edge max(U[taper], taper_frac * side,
+    min_taper) cells wide. Overlapping scars take the minimum factor, not the
+    product. The edge scales with the grid like the reference's blur, whose
+    kernel spans 10% of the image (deepx/utils_scars.py:402-426), never
+    narrower than 12 cells, and the scale stays >= 0.1, the floor of the
+    reference's own random D range [1e-4, 1e-3] (deepx/generate.py:80). These
+    keep the dynamics bounded: the reference's non-conservative
+    D (u_xx + u_yy) + D_x u_x + D_y u_y with its stencils has growing modes
+    where D changes steeply (edges of ~6 cells on 64^2 grids diverge for
+    PARAMSET_1E, 2, 5, 6 even in FP64), which excited tissue does not damp.
     """
     H, W = shape
     s = min(H, W) / 1024.0
@@ -88,7 +88,7 @@ def scar_map(shape, rng, n_min=1, n_max=5, axes=(20, 200), scale=(0.1, 0.2), bas
         b = max(2.0, rng.uniform(*axes) * s / 2)
         th = rng.uniform(0, np.pi)
         f = rng.uniform(*scale)
-        tw = max(rng.uniform(*taper), taper_frac * min(H, W))  # edge width, cells
+        tw = max(rng.uniform(*taper), taper_frac * min(H, W), min_taper)  # edge width, cells
         dy, dx = yy - cy, xx - cx
         xr = dx * np.cos(th) + dy * np.sin(th)
         yr = -dx * np.sin(th) + dy * np.cos(th)

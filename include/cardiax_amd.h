@@ -72,6 +72,23 @@ int32_t fk_abi_version(void);
  * previous setting, or FK_ERR_ARG for an unknown variant. */
 int32_t fk_set_forward_variant(int32_t variant);
 
+/* Autotuner of the forward step kernel. No reference counterpart (the
+ * reference leaves kernel choice to XLA). Times every variant except the
+ * plain one over n_steps Euler steps (1..100000) of an H x W grid on `stream`
+ * (synchronising it; scratch memory is allocated and freed here) and records
+ * the fastest for that shape: automatic launches (variant 0) on an H x W grid
+ * use it from then on, process-wide. times_ms (may be NULL) receives one time
+ * per variant index (-1 for variants not timed); best (may be NULL) the
+ * chosen variant. */
+int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
+                            float* times_ms, int32_t* best);
+
+/* The variant tuned for an H x W grid, or 0 when that shape was not tuned. */
+int32_t fk_autotune_lookup(int32_t H, int32_t W);
+
+/* Forgets every tuned shape. */
+void fk_autotune_reset(void);
+
 /* cardiax/solve.py:18-23 -- v = w = 1, u = 0 over n_cells cells. */
 int32_t fk_init(float* v, float* w, float* u, int64_t n_cells, void* stream);
 
@@ -223,6 +240,18 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out);
  * use_graph != 0 replays a recorded hipGraph of 32 steps (one graph launch
  * per 32 steps for the whole batch), otherwise one launch per step. */
 int32_t fk_batch_run(fk_batch* b, int64_t n_steps, int32_t use_graph, void* stream);
+
+/* Autotuner of the batch: times n_steps (1..10000) batched steps with bands
+ * of 8 and of 16 rows on `stream` (synchronising it) and keeps the faster work
+ * list for later runs. Overwrites only the batch's internal ping-pong pools,
+ * never a simulation's state. times_ms (2 floats, may be NULL) and best_rows
+ * (may be NULL) receive the timings and the chosen band height. */
+int32_t fk_batch_autotune(fk_batch* b, int32_t n_steps, void* stream, float* times_ms,
+                          int32_t* best_rows);
+
+/* Pins the band height (8 or 16 rows) of later runs, as the autotuner would;
+ * waits for the device first. */
+int32_t fk_batch_set_rows(fk_batch* b, int32_t rows);
 
 /* Number of (simulation, strip, band) tiles of one batch step. */
 int32_t fk_batch_n_items(const fk_batch* b);

@@ -1,0 +1,88 @@
+"""GPU tests of the autotuners (fk_autotune_forward, fk_batch_autotune).
+
+Tuning only changes which kernel runs: results must stay within the parity
+bar of the FP64 oracle (forward) and bit-identical to per-simulation runs
+(batch), and the tuned choice must be what automatic launches use.
+"""
+import numpy as np
+import pytest
+import torch
+
+from cardiax_amd import _lib, autotune, solve
+from cardiax_amd import workloads as wl
+from cardiax_amd.batch import Batch
+from oracle import c_oracle
+from oracle import fk_oracle as o
+
+from test_gpu_batch import _sims
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_autotune_records_and_is_used(gpu):
+    shape = (96, 130)
+    autotune.reset()
+    try:
+        assert autotune.lookup(shape) == "auto"
+        res = autotune.tune(shape, n_steps=20)
+        timed = set(res["times_ms"])
+        assert timed == {"stream8", "stream16", "stream32", "tile16", "tile32", "tb64x2",
+                         "tb64x4", "tb32x2", "tb32x4"}
+        assert all(t > 0 for t in res["times_ms"].values())
+        assert res["best"] == min(res["times_ms"], key=res["times_ms"].get)
+        assert autotune.lookup(shape) == res["best"]
+        assert autotune.lookup((97, 130)) == "auto"
+        # an automatic solve on the tuned shape stays within the parity bar
+        sim = wl.c1_inputs(3, shape=shape, steps=150)
+        stims = [(s.protocol, s.field) for s in sim.stimuli]
+        ostims = [o.Stimulus(o.Protocol(*s.protocol), s.field) for s in sim.stimuli]
+        g = solve._forward_euler(solve.init(shape), 0, 150, sim.params, sim.D, stims, 0.01, 0.01)
+        r = c_oracle.forward_euler(o.init(shape), 0, 150, sim.params, sim.D, ostims, 0.01, 0.01)
+        for a, b in zip(g, r):
+            assert np.abs(a.cpu().numpy() - b).max() <= 1e-4
+    finally:
+        autotune.reset()
+    assert autotune.lookup(shape) == "auto"
+
+
+def test_autotune_rejects_bad_arguments(gpu):
+    L = _lib.lib()
+    assert L.fk_autotune_forward(2, 50, 10, None, None, None) == _lib.FK_ERR_ARG
+    assert L.fk_autotune_forward(50, 50, 0, None, None, None) == _lib.FK_ERR_ARG
+
+
+def test_batch_autotune_keeps_results_bitwise(gpu):
+    sims = _sims(2)
+    n = 80
+    with Batch(sims) as b:
+        before = [torch.stack(list(s)).clone() for s in b.states]
+        res = b.autotune(n_steps=8)
+        assert res["rows"] in (8, 16) and all(t > 0 for t in res["times_ms"])
+        # tuning never touches the simulations' states
+        for x, s in zip(before, b.states):
+            assert torch.equal(x, torch.stack(list(s)))
+        b.run(n)
+        torch.cuda.synchronize()
+        for s, got in zip(sims, b.states):
+            ref = solve._forward_euler(s.state, s.t, s.t + n, s.params, s.diffusivity,
+                                       s.stimuli, s.dt, s.dx)
+            for a, r in zip(got, ref):
+                assert torch.equal(a, r)
+
+
+@pytest.mark.parametrize("rows", [8, 16])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_band_heights_match_per_simulation_runs(gpu, rows, use_graph):
+    sims = _sims(3)
+    n = 90
+    with Batch(sims) as b:
+        b.set_rows(rows)
+        b.run(n, use_graph=use_graph)
+        torch.cuda.synchronize()
+        for s, got in zip(sims, b.states):
+            ref = solve._forward_euler(s.state, s.t, s.t + n, s.params, s.diffusivity,
+                                       s.stimuli, s.dt, s.dx)
+            for a, r in zip(got, ref):
+                assert torch.equal(a, r)
+        with pytest.raises(ValueError):
+            b.set_rows(12)

@@ -1,0 +1,76 @@
+"""Growth of the fp32 GPU error against the FP64 oracle over long horizons.
+
+The fp32 solver and the FP64 restatement of the reference (oracle/fk_oracle.c)
+run the same inputs; at each horizon the test prints the max-abs and RMS
+difference of u and the activation-time mismatch (first crossing of
+u >= 0.5, sampled every `every` steps). The parity bar (max-abs <= 1e-4) is
+asserted at 1e3 steps; longer horizons are reported (the spiral regime is
+chaotic, so the fp32/FP64 difference grows there by construction) and only
+checked for finiteness and boundedness.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from cardiax_amd import solve
+from cardiax_amd import workloads as wl
+from oracle import c_oracle
+from oracle import fk_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def npy(t):
+    return t.detach().cpu().numpy()
+
+
+def _growth(sim, horizons, every):
+    stims = [(s.protocol, s.field) for s in sim.stimuli]
+    ostims = [o.Stimulus(o.Protocol(*s.protocol), s.field) for s in sim.stimuli]
+    g, r = solve.init(sim.shape), o.init(sim.shape)
+    ag = np.full(sim.shape, np.inf)
+    ar = np.full(sim.shape, np.inf)
+    rows, t = [], 0
+    for h in horizons:
+        while t < h:
+            t1 = min(h, t + every)
+            g = solve._forward_euler(g, t, t1, sim.params, sim.D, stims, sim.dt, sim.dx)
+            r = c_oracle.forward_euler(r, t, t1, sim.params, sim.D, ostims, sim.dt, sim.dx)
+            t = t1
+            gu = npy(g.u)
+            for a, u in ((ag, gu), (ar, r.u)):
+                new = np.isinf(a) & (u >= 0.5)
+                a[new] = t
+        d = npy(g.u).astype(np.float64) - r.u
+        both = np.isfinite(ag) & np.isfinite(ar)
+        rows.append({
+            "steps": h,
+            "u_maxabs": float(np.abs(d).max()),
+            "u_rms": float(np.sqrt(np.mean(d * d))),
+            "u_range": float(np.abs(r.u).max()),
+            "activated_cells": int(both.sum()),
+            "activation_set_mismatch": int((np.isfinite(ag) != np.isfinite(ar)).sum()),
+            "activation_maxdiff_steps": float(np.abs(ag[both] - ar[both]).max()) if both.any()
+            else 0.0,
+        })
+    return rows
+
+
+def test_error_growth_c1_128():
+    sim = wl.c1_inputs(0, steps=100_000)
+    rows = _growth(sim, [1_000, 10_000, 100_000], every=50)
+    print("C1 128x128 error growth:", json.dumps(rows))
+    assert rows[0]["u_maxabs"] <= 1e-4
+    for r in rows:
+        assert np.isfinite(r["u_maxabs"]) and r["u_maxabs"] < 1.0
+
+
+def test_error_growth_spiral_256():
+    sim = wl.c2_inputs(0, shape=(256, 256), steps=20_000)
+    rows = _growth(sim, [1_000, 2_000, 5_000, 10_000, 20_000], every=50)
+    print("scarred 256x256 S1-S2 error growth (S2 at step %d):" % sim.meta["s2_start"],
+          json.dumps(rows))
+    assert rows[0]["u_maxabs"] <= 1e-4
+    for r in rows:
+        assert np.isfinite(r["u_maxabs"])
