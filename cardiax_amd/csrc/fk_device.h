@@ -86,6 +86,19 @@ __device__ __forceinline__ float d1_backward(float a3, float a2, float a1, float
   return -(FK_C_E0 * a0 + FK_C_E1 * a1 + FK_C_E2 * a2 + FK_C_E3 * a3);
 }
 
+// The same three rules times 12, with exact integer coefficients: kernels
+// that keep derivatives in this scaling (12 per application) fold 1/144 into
+// the grid factor and save one multiply per difference.
+__device__ __forceinline__ float s1_interior(float m2, float m1, float p1, float p2) {
+  return fmaf(8.0f, p1 - m1, m2 - p2);
+}
+__device__ __forceinline__ float s1_forward(float a0, float a1, float a2, float a3) {
+  return -22.0f * a0 + 36.0f * a1 - 18.0f * a2 + 4.0f * a3;
+}
+__device__ __forceinline__ float s1_backward(float a3, float a2, float a1, float a0) {
+  return -(-22.0f * a0 + 36.0f * a1 - 18.0f * a2 + 4.0f * a3);
+}
+
 // First derivative (no spacing) at original index q in [-1, N] of an axis of
 // N cells that the reference edge-pads by one: padded index q+1, one-sided
 // rules on padded indices {0,1} and {N, N+1}. `f(r)` returns the padded value

@@ -28,25 +28,28 @@ namespace fk {
 constexpr int kStripOut = 56;  // output columns per wave
 
 // Lane l receives the value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1).
-// The lane without a source keeps 0; it is a halo lane whose result is unused.
+// The lane without a source reads 0 (bound_ctrl): it is a halo lane whose
+// result is unused. With no "old" operand the move needs no initialisation
+// and can fold into the consuming instruction as a DPP source.
 __device__ __forceinline__ float from_left(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x130, 0xf, 0xf, true));
 }
 
-// First difference across columns of a value held one per lane. GENERAL:
-// the lane's column `col` selects the rule, as d1_general does.
+// First difference across columns (times 12, s1_* rules) of a value held one
+// per lane. GENERAL: the lane's column `col` selects the rule, as d1_general
+// does.
 template <bool GENERAL>
 __device__ __forceinline__ float d1_lanes(float x, int col, int W) {
   const float m1 = from_left(x), p1 = from_right(x);
   const float m2 = from_left(m1), p2 = from_right(p1);
-  const float ctr = d1_interior(m2, m1, p1, p2);
+  const float ctr = s1_interior(m2, m1, p1, p2);
   if (!GENERAL) return ctr;
   const float m3 = from_left(m2), p3 = from_right(p2);
-  const float fwd = d1_forward(x, p1, p2, p3);
-  const float bwd = d1_backward(m3, m2, m1, x);
+  const float fwd = s1_forward(x, p1, p2, p3);
+  const float bwd = s1_backward(m3, m2, m1, x);
   return col <= 0 ? fwd : (col >= W - 1 ? bwd : ctr);
 }
 
@@ -56,11 +59,11 @@ template <bool GENERAL>
 __device__ __forceinline__ float d2_lanes(float g, int col, int W) {
   const float m1 = from_left(g), p1 = from_right(g);
   const float m2 = from_left(m1), p2 = from_right(p1);
-  const float ctr = d1_interior(m2, m1, p1, p2);
+  const float ctr = s1_interior(m2, m1, p1, p2);
   if (!GENERAL) return ctr;
   const float m3 = from_left(m2), p3 = from_right(p2);
-  const float fwd = d1_forward(g, p1, p2, p3);
-  const float bwd = d1_backward(m3, m2, m1, g);
+  const float fwd = s1_forward(g, p1, p2, p3);
+  const float bwd = s1_backward(m3, m2, m1, g);
   return col == 0 ? fwd : (col == W - 1 ? bwd : ctr);
 }
 
@@ -134,17 +137,19 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
     ww[k] = ld(row(w, rs + k), off);
   }
 
-  // u_x at rows q = rs-2 .. rs+R+1
+  // 12 u_x at rows q = rs-2 .. rs+R+1 (every difference below is scaled by
+  // 12; the diffusion term divides by 144 once)
+  const float k2 = inv_dx2 * (1.0f / 144.0f);
   float gx[R + 4];
 #pragma unroll
   for (int k = 0; k < R + 4; ++k) {
     const int q = rs - 2 + k;
     if (XG && k <= 2 && q <= 0)  // forward rule reads u(q..q+3); q <= 0 forces k <= 2
-      gx[k] = d1_forward(uu[k + 2], uu[k + 3], uu[k + 4], uu[k + 5]);
+      gx[k] = s1_forward(uu[k + 2], uu[k + 3], uu[k + 4], uu[k + 5]);
     else if (XG && k >= 3 && q >= H - 1)  // backward rule reads u(q-3..q)
-      gx[k] = d1_backward(uu[k - 1], uu[k], uu[k + 1], uu[k + 2]);
+      gx[k] = s1_backward(uu[k - 1], uu[k], uu[k + 1], uu[k + 2]);
     else
-      gx[k] = d1_interior(uu[k], uu[k + 1], uu[k + 3], uu[k + 4]);
+      gx[k] = s1_interior(uu[k], uu[k + 1], uu[k + 3], uu[k + 4]);
   }
 
 #pragma unroll
@@ -153,19 +158,19 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
     const float uc = uu[k + 4], Dc = dd[k + 2], gxc = gx[k + 2];
     float uxx, dDx;
     if (XG && k == 0 && i == 0) {  // i = rs + k = 0 forces k = 0
-      uxx = d1_forward(gx[k + 2], gx[k + 3], gx[k + 4], gx[k + 5]);
-      dDx = d1_forward(dd[k + 2], dd[k + 3], dd[k + 4], dd[k + 5]);
+      uxx = s1_forward(gx[k + 2], gx[k + 3], gx[k + 4], gx[k + 5]);
+      dDx = s1_forward(dd[k + 2], dd[k + 3], dd[k + 4], dd[k + 5]);
     } else if (XG && k >= 1 && i == H - 1) {
-      uxx = d1_backward(gx[k - 1], gx[k], gx[k + 1], gx[k + 2]);
-      dDx = d1_backward(dd[k - 1], dd[k], dd[k + 1], dd[k + 2]);
+      uxx = s1_backward(gx[k - 1], gx[k], gx[k + 1], gx[k + 2]);
+      dDx = s1_backward(dd[k - 1], dd[k], dd[k + 1], dd[k + 2]);
     } else {
-      uxx = d1_interior(gx[k], gx[k + 1], gx[k + 3], gx[k + 4]);
-      dDx = d1_interior(dd[k], dd[k + 1], dd[k + 3], dd[k + 4]);
+      uxx = s1_interior(gx[k], gx[k + 1], gx[k + 3], gx[k + 4]);
+      dDx = s1_interior(dd[k], dd[k + 1], dd[k + 3], dd[k + 4]);
     }
     const float gyc = d1_lanes<YG>(uc, col, W);
     const float uyy = d2_lanes<YG>(gyc, col, W);
     const float dDy = d1_lanes<YG>(Dc, col, W);
-    const float del_u = (Dc * (uxx + uyy) + dDx * gxc + dDy * gyc) * inv_dx2;
+    const float del_u = (Dc * (uxx + uyy) + dDx * gxc + dDy * gyc) * k2;
     float dv, dw, jion;
     reaction(c, vv[k], ww[k], uc, dv, dw, jion);
     if (STIM) {
