@@ -115,6 +115,7 @@ k_batch_step(const SimDesc* __restrict__ sims, const TileItem* __restrict__ item
   const BatchStimView sv{S.stim, uni(S.n_stim), mask};
   Coef c = S.c;
   c.V_c = uni(c.V_c); c.V_v = uni(c.V_v); c.V_csi = uni(c.V_csi); c.kk = uni(c.kk);
+  c.kk_off = uni(c.kk_off);
   c.inv_tvp = uni(c.inv_tvp); c.inv_tv1m = uni(c.inv_tv1m); c.inv_tv2m = uni(c.inv_tv2m);
   c.inv_twp = uni(c.inv_twp); c.inv_twm = uni(c.inv_twm); c.inv_td = uni(c.inv_td);
   c.inv_t0 = uni(c.inv_t0); c.inv_tr = uni(c.inv_tr); c.inv_tsi = uni(c.inv_tsi);
@@ -130,7 +131,7 @@ k_batch_step(const SimDesc* __restrict__ sims, const TileItem* __restrict__ item
   const float dt = uni(S.dt), inv_dx2 = uni(S.inv_dx2);
   const int sel = item.flags | (mask ? 4 : 0);
 #define FK_BT(X, Y, ST)                                                                    \
-  stream_tile<R, true, X, Y, ST>(c, Dp, i0, i1, i2, H, W, sv, dt, inv_dx2, o0, o1, o2,       \
+  stream_tile<R, true, X, Y, ST>(c, Dp, nullptr, i0, i1, i2, H, W, sv, dt, inv_dx2, o0, o1, o2, \
                                        item.rs, item.cs)
   switch (sel) {
     case 0: FK_BT(false, false, false); break;

@@ -16,7 +16,8 @@ namespace fk {
 // rounded once to float). Reciprocals replace the reference's divisions.
 struct Coef {
   float V_c, V_v, V_csi;
-  float kk;  // -2 k log2(e): exp(-2k(u - V_csi)) = exp2((u - V_csi) * kk)
+  float kk;      // -2 k log2(e): exp(-2k(u - V_csi)) = exp2(u * kk + kk_off)
+  float kk_off;  // -V_csi * kk
   float inv_tvp, inv_tv1m, inv_tv2m, inv_twp, inv_twm;
   float inv_td, inv_t0, inv_tr, inv_tsi, inv_cm;
 };
@@ -36,23 +37,29 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) {
 // no cancellation for any z and saturates to exactly 0 / 1 (exp2 -> inf / 0),
 // which the MAXFLOAT sentinel sets (cardiax/params.py:43-58,78-93) rely on.
 __device__ __forceinline__ float half_one_plus_tanh(const Coef& c, float u) {
-  const float e = __builtin_amdgcn_exp2f((u - c.V_csi) * c.kk);
+#pragma clang fp contract(off)
+  const float e = __builtin_amdgcn_exp2f(fmaf(u, c.kk, c.kk_off));
   return __builtin_amdgcn_rcpf(1.0f + e);
 }
 
 // Gates and ionic currents, cardiax/solve.py:35-42,57-58.
 // p = [u >= V_c], q = [u >= V_v]; outputs dv, dw and j_ion (before stimulus).
 // Both sides of every gate are evaluated and selected (no divergent branch).
+// Contraction is off in the shared step code: every kernel that inlines it
+// rounds identically, so batched and single-grid runs agree bit for bit.
 __device__ __forceinline__ void reaction(const Coef& c, float v, float w, float u, float& dv,
                                          float& dw, float& jion) {
+#pragma clang fp contract(off)
   const bool p = u >= c.V_c;
-  const float jsi = -(w * half_one_plus_tanh(c, u)) * c.inv_tsi;
-  const float jfi = p ? -v * (u - c.V_c) * (1.0f - u) * c.inv_td : 0.0f;
-  const float jso = p ? c.inv_tr : u * c.inv_t0;
+  // j_fi + j_so: p ? -v (u - V_c)(1 - u) / tau_d + 1 / tau_r : u / tau_0
+  const float a = (u - c.V_c) * (1.0f - u);
+  const float fo = p ? fmaf(-(v * a), c.inv_td, c.inv_tr) : u * c.inv_t0;
+  // + j_si = -w S / tau_si, S = (1 + tanh(k (u - V_csi))) / 2
+  const float s = fmaf(-(w * c.inv_tsi), half_one_plus_tanh(c, u), fo);
   const float inv_tvm = (u >= c.V_v) ? c.inv_tv2m : c.inv_tv1m;
   dv = p ? -v * c.inv_tvp : (1.0f - v) * inv_tvm;
   dw = p ? -w * c.inv_twp : (1.0f - w) * c.inv_twm;
-  jion = -(jfi + jso + jsi) * c.inv_cm;
+  jion = -s * c.inv_cm;
 }
 
 // Stimulus replacement of cardiax/solve.py:257-271 at one cell.
@@ -90,13 +97,16 @@ __device__ __forceinline__ float d1_backward(float a3, float a2, float a1, float
 // that keep derivatives in this scaling (12 per application) fold 1/144 into
 // the grid factor and save one multiply per difference.
 __device__ __forceinline__ float s1_interior(float m2, float m1, float p1, float p2) {
+#pragma clang fp contract(off)
   return fmaf(8.0f, p1 - m1, m2 - p2);
 }
 __device__ __forceinline__ float s1_forward(float a0, float a1, float a2, float a3) {
-  return -22.0f * a0 + 36.0f * a1 - 18.0f * a2 + 4.0f * a3;
+#pragma clang fp contract(off)
+  return fmaf(4.0f, a3, fmaf(-18.0f, a2, fmaf(36.0f, a1, -22.0f * a0)));
 }
 __device__ __forceinline__ float s1_backward(float a3, float a2, float a1, float a0) {
-  return -(-22.0f * a0 + 36.0f * a1 - 18.0f * a2 + 4.0f * a3);
+#pragma clang fp contract(off)
+  return -fmaf(4.0f, a3, fmaf(-18.0f, a2, fmaf(36.0f, a1, -22.0f * a0)));
 }
 
 // First derivative (no spacing) at original index q in [-1, N] of an axis of

@@ -206,6 +206,29 @@ k_tile(Coef c, const float* __restrict__ D, const float* __restrict__ v,
 // ---------------------------------------------------------------------------
 // Small kernels: init, gradient, stimulate
 // ---------------------------------------------------------------------------
+// Static diffusion coefficients of one cell, precomputed once per solve:
+// (D, 12 D_x, 12 D_y) with the reference's rules (edge padding, one-sided
+// differences at the two outermost padded entries), each times
+// inv_dx2 / 144, so that the step's diffusion term is
+// x (12^2 u_xx + 12^2 u_yy) + y (12 u_x) + z (12 u_y).
+__global__ void __launch_bounds__(256)
+k_stream_coef(const float* __restrict__ D, int H, int W, float k2, float4* __restrict__ cf) {
+  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= H || j >= W) return;
+  auto at = [&](int r, int q) {
+    return D[(size_t)clampi(r, 0, H - 1) * W + clampi(q, 0, W - 1)];
+  };
+  float dx, dy;
+  if (i == 0) dx = s1_forward(at(0, j), at(1, j), at(2, j), at(3, j));
+  else if (i == H - 1) dx = s1_backward(at(i - 3, j), at(i - 2, j), at(i - 1, j), at(i, j));
+  else dx = s1_interior(at(i - 2, j), at(i - 1, j), at(i + 1, j), at(i + 2, j));
+  if (j == 0) dy = s1_forward(at(i, 0), at(i, 1), at(i, 2), at(i, 3));
+  else if (j == W - 1) dy = s1_backward(at(i, j - 3), at(i, j - 2), at(i, j - 1), at(i, j));
+  else dy = s1_interior(at(i, j - 2), at(i, j - 1), at(i, j + 1), at(i, j + 2));
+  cf[(size_t)i * W + j] = make_float4(at(i, j) * k2, dx * k2, dy * k2, 0.0f);
+}
+
 __global__ void k_fill(float* a, size_t n, float x) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -265,8 +288,12 @@ enum Variant : int {
   kTb64x4 = 8,   // 64 x 64 tiles, 4 steps per launch
   kTb32x2 = 9,   // 32 x 32 tiles, 2 steps per launch
   kTb32x4 = 10,  // 32 x 32 tiles, 4 steps per launch
-  kNumVariants = 11
+  kStreamC8 = 11,   // streaming bands of 8 / 16 rows with the static diffusion
+  kStreamC16 = 12,  // coefficients precomputed once per solve
+  kNumVariants = 13
 };
+
+static bool is_pc(int v) { return v == kStreamC8 || v == kStreamC16; }
 
 static bool is_tb(int v) { return v >= kTb64x2 && v <= kTb32x4; }
 
@@ -311,6 +338,20 @@ static void launch_stream(bool euler, const Coef& c, const float* D, const float
                                              ou, g);
 }
 
+template <int R>
+static void launch_stream_c(const Coef& c, const float4* cf, const float* v, const float* w,
+                            const float* u, int H, int W, const StimArgs& sa, float dt,
+                            float* ov, float* ow, float* ou, hipStream_t s) {
+  const StreamGeom g = make_stream_geom<R>(H, W);
+  k_stream_c<R><<<g.blocks, 256, 0, s>>>(c, cf, v, w, u, H, W, sa, dt, ov, ow, ou, g);
+}
+
+// Forward workspace: ping-pong state (3 fields), then, 256-byte aligned, the
+// precomputed diffusion coefficients (one float4 per cell).
+static size_t coef_offset(int H, int W) {
+  return (3 * (size_t)H * W * sizeof(float) + 255) / 256 * 256;
+}
+
 template <int TH>
 static void launch_tile(bool euler, const Coef& c, const float* D, const float* v,
                         const float* w, const float* u, int H, int W, const StimArgs& sa,
@@ -341,6 +382,7 @@ static hipError_t launch_step(int variant, bool euler, const Coef& c, const floa
     case kTile16: launch_tile<16>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
     case kTile32: launch_tile<32>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
     case kStream32: launch_stream<32>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
+    case kStreamC16:  // single steps: no coefficient precompute to amortise
     case kStream16: launch_stream<16>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
     case kStream8:
     default: launch_stream<8>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
@@ -404,7 +446,15 @@ static int32_t forward_loop(int variant, const fk_params* params, const float* D
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
   float* wk = (float*)work;
   float* bufs[2][3] = {{v_out, w_out, u_out}, {wk, wk + n, wk + 2 * n}};
-  const int v = resolve_for(variant, H, W);
+  int v = resolve_for(variant, H, W);
+  if (is_pc(v) && steps < 2) v = (v == kStreamC8) ? kStream8 : kStream16;  // nothing to amortise
+  float4* cf = nullptr;
+  if (is_pc(v)) {
+    cf = (float4*)((char*)work + coef_offset(H, W));
+    const dim3 grid((W + 63) / 64, (H + 3) / 4);
+    k_stream_coef<<<grid, 256, 0, s>>>(D, H, W, (float)(inv_dx2 / 144.0), cf);
+    HIP_CHECK(hipGetLastError());
+  }
   const int depth = is_tb(v) ? tb_depth(v) : 1;
   const int64_t launches = (steps + depth - 1) / depth;
   // Ping-pong so that the last launch lands in the output buffers.
@@ -421,6 +471,13 @@ static int32_t forward_loop(int variant, const fk_params* params, const float* D
       for (int j = 0; j < 8; ++j) st.act[j] = j < ns ? sched.mask(t + (double)(k + j)) : 0u;
       HIP_CHECK(launch_tb_variant(v, c, D, src[0], src[1], src[2], H, W, st, ns, dt, inv_dx2,
                                   d[0], d[1], d[2], s));
+    } else if (is_pc(v)) {
+      const StimArgs& sa = sched.at(t + (double)k);
+      if (v == kStreamC8)
+        launch_stream_c<8>(c, cf, src[0], src[1], src[2], H, W, sa, dt, d[0], d[1], d[2], s);
+      else
+        launch_stream_c<16>(c, cf, src[0], src[1], src[2], H, W, sa, dt, d[0], d[1], d[2], s);
+      HIP_CHECK(hipGetLastError());
     } else {
       const StimArgs& sa = sched.at(t + (double)k);
       HIP_CHECK(launch_step(v, true, c, D, src[0], src[1], src[2], H, W, sa, dt, inv_dx2,
@@ -495,7 +552,7 @@ int32_t fk_step_euler(const fk_params* params, const float* D, const float* v,
 
 size_t fk_forward_workspace_bytes(int32_t H, int32_t W) {
   if (H <= 0 || W <= 0) return 0;
-  return 3 * (size_t)H * (size_t)W * sizeof(float);
+  return coef_offset(H, W) + (size_t)H * (size_t)W * sizeof(float4);
 }
 
 int32_t fk_forward_euler(const fk_params* params, const float* D, const float* v_in,
@@ -515,9 +572,11 @@ int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
     return set_error(FK_ERR_ARG, "n_steps must be in [1, 100000], got %d", n_steps);
   hipStream_t s = (hipStream_t)stream;
   const size_t n = (size_t)H * W;
+  const size_t wb = fk_forward_workspace_bytes(H, W);
   float* mem = nullptr;
-  HIP_CHECK(hipMalloc((void**)&mem, 10 * n * sizeof(float)));
-  float *D = mem, *x = mem + n, *y = mem + 4 * n, *wk = mem + 7 * n;
+  HIP_CHECK(hipMalloc((void**)&mem, 7 * n * sizeof(float) + 256 + wb));
+  float *D = mem, *x = mem + n, *y = mem + 4 * n;
+  float* wk = (float*)(((uintptr_t)(mem + 7 * n) + 255) / 256 * 256);
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
   k_fill<<<blocks, 256, 0, s>>>(D, n, 1e-3f);
   k_init<<<blocks, 256, 0, s>>>(x, x + n, x + 2 * n, (int64_t)n);
@@ -529,11 +588,10 @@ int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
   if (e == hipSuccess) e = hipEventCreate(&e1);
   int best_v = kStream8;
   float best_ms = 0.0f;
-  const int cand[] = {kStream8, kStream16, kStream32, kTile16, kTile32,
-                      kTb64x2,  kTb64x4,   kTb32x2,   kTb32x4};
+  const int cand[] = {kStream8, kStream16, kStream32, kStreamC8, kStreamC16, kTile16,
+                      kTile32,  kTb64x2,   kTb64x4,   kTb32x2,   kTb32x4};
   if (times_ms)
     for (int k = 0; k < kNumVariants; ++k) times_ms[k] = -1.0f;
-  const size_t wb = 3 * n * sizeof(float);
   for (int v : cand) {
     if (e != hipSuccess) break;
     // warm-up launch, then the timed run

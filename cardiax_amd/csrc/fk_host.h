@@ -186,6 +186,7 @@ inline Coef make_coef(const fk_params& p) {
   c.V_v = p.V_v;
   c.V_csi = p.V_csi;
   c.kk = (float)(-2.0 * (double)p.k * log2e);
+  c.kk_off = (float)(2.0 * (double)p.k * log2e * (double)p.V_csi);
   c.inv_tvp = (float)(1.0 / (double)p.tau_v_plus);
   c.inv_tv1m = (float)(1.0 / (double)p.tau_v1_minus);
   c.inv_tv2m = (float)(1.0 / (double)p.tau_v2_minus);

@@ -43,6 +43,7 @@ __device__ __forceinline__ float from_right(float x) {
 // does.
 template <bool GENERAL>
 __device__ __forceinline__ float d1_lanes(float x, int col, int W) {
+#pragma clang fp contract(off)
   const float m1 = from_left(x), p1 = from_right(x);
   const float m2 = from_left(m1), p2 = from_right(p1);
   const float ctr = s1_interior(m2, m1, p1, p2);
@@ -57,6 +58,7 @@ __device__ __forceinline__ float d1_lanes(float x, int col, int W) {
 // lane (cardiax/solve.py:51-52 applied to u_y): one-sided at columns 0, W-1.
 template <bool GENERAL>
 __device__ __forceinline__ float d2_lanes(float g, int col, int W) {
+#pragma clang fp contract(off)
   const float m1 = from_left(g), p1 = from_right(g);
   const float m2 = from_left(m1), p2 = from_right(p1);
   const float ctr = s1_interior(m2, m1, p1, p2);
@@ -97,6 +99,12 @@ __device__ __forceinline__ float ld(const float* rowp, uint32_t off) {
 __device__ __forceinline__ void st(float* rowp, uint32_t off, float x) {
   *(float __attribute__((address_space(1)))*)((gbytes)rowp + off) = x;
 }
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4(const float4* rowp, uint32_t off) {
+  const f32x4 x = *(const f32x4 __attribute__((address_space(1)))*)((gcbytes)rowp + off);
+  return make_float4(x.x, x.y, x.z, x.w);
+}
+
 
 // Stimulus fields seen by a tile: count(), on(m) (field m active this step)
 // and field(m), in list order. This view reads the active list passed as a
@@ -108,14 +116,18 @@ struct StimListView {
   __device__ const float* field(int m) const { return sa.f[m]; }
 };
 
-template <int R, bool EULER, bool XG, bool YG, bool STIM, class SV>
+// PC: the static diffusion coefficients come precomputed from `cf`
+// (k_stream_coef) instead of being derived from D each step.
+template <int R, bool EULER, bool XG, bool YG, bool STIM, class SV, bool PC = false>
 __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restrict__ D,
+                                            const float4* __restrict__ cf,
                                             const float* __restrict__ v,
                                             const float* __restrict__ w,
                                             const float* __restrict__ u, int H, int W,
                                             const SV& sv, float dt, float inv_dx2,
                                             float* __restrict__ o_v, float* __restrict__ o_w,
                                             float* __restrict__ o_u, int rs, int cs) {
+#pragma clang fp contract(off)  // explicit fmaf only: identical rounding in every kernel
   const int lane = threadIdx.x & 63;
   const int col = cs - 4 + lane;                           // this lane's column
   const uint32_t off = (uint32_t)clampi(col, 0, W - 1) * 4u;  // clamped = edge padding
@@ -124,13 +136,20 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
     return a + (size_t)clampi(r, 0, H - 1) * (size_t)W;
   };
 
-  float uu[R + 8];  // u rows rs-4 .. rs+R+3
-  float dd[R + 4];  // D rows rs-2 .. rs+R+1
-  float vv[R], ww[R];
+  float uu[R + 8];           // u rows rs-4 .. rs+R+3
+  float dd[PC ? 1 : R + 4];  // D rows rs-2 .. rs+R+1
+  float4 cc[PC ? R : 1];     // precomputed coefficients of rows rs .. rs+R-1
 #pragma unroll
   for (int k = 0; k < R + 8; ++k) uu[k] = ld(row(u, rs - 4 + k), off);
+  if constexpr (PC) {
 #pragma unroll
-  for (int k = 0; k < R + 4; ++k) dd[k] = ld(row(D, rs - 2 + k), off);
+    for (int k = 0; k < R; ++k)
+      cc[k] = ld4(cf + (size_t)clampi(rs + k, 0, H - 1) * (size_t)W, off * 4u);
+  } else {
+#pragma unroll
+    for (int k = 0; k < R + 4; ++k) dd[k] = ld(row(D, rs - 2 + k), off);
+  }
+  float vv[R], ww[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     vv[k] = ld(row(v, rs + k), off);
@@ -155,22 +174,31 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int i = rs + k;
-    const float uc = uu[k + 4], Dc = dd[k + 2], gxc = gx[k + 2];
-    float uxx, dDx;
-    if (XG && k == 0 && i == 0) {  // i = rs + k = 0 forces k = 0
+    const float uc = uu[k + 4], gxc = gx[k + 2];
+    float uxx;
+    if (XG && k == 0 && i == 0)  // i = rs + k = 0 forces k = 0
       uxx = s1_forward(gx[k + 2], gx[k + 3], gx[k + 4], gx[k + 5]);
-      dDx = s1_forward(dd[k + 2], dd[k + 3], dd[k + 4], dd[k + 5]);
-    } else if (XG && k >= 1 && i == H - 1) {
+    else if (XG && k >= 1 && i == H - 1)
       uxx = s1_backward(gx[k - 1], gx[k], gx[k + 1], gx[k + 2]);
-      dDx = s1_backward(dd[k - 1], dd[k], dd[k + 1], dd[k + 2]);
-    } else {
+    else
       uxx = s1_interior(gx[k], gx[k + 1], gx[k + 3], gx[k + 4]);
-      dDx = s1_interior(dd[k], dd[k + 1], dd[k + 3], dd[k + 4]);
-    }
     const float gyc = d1_lanes<YG>(uc, col, W);
     const float uyy = d2_lanes<YG>(gyc, col, W);
-    const float dDy = d1_lanes<YG>(Dc, col, W);
-    const float del_u = (Dc * (uxx + uyy) + dDx * gxc + dDy * gyc) * k2;
+    float del_u;
+    if constexpr (PC) {
+      del_u = fmaf(cc[k].z, gyc, fmaf(cc[k].y, gxc, cc[k].x * (uxx + uyy)));
+    } else {
+      const float Dc = dd[k + 2];
+      float dDx;
+      if (XG && k == 0 && i == 0)
+        dDx = s1_forward(dd[k + 2], dd[k + 3], dd[k + 4], dd[k + 5]);
+      else if (XG && k >= 1 && i == H - 1)
+        dDx = s1_backward(dd[k - 1], dd[k], dd[k + 1], dd[k + 2]);
+      else
+        dDx = s1_interior(dd[k], dd[k + 1], dd[k + 3], dd[k + 4]);
+      const float dDy = d1_lanes<YG>(Dc, col, W);
+      del_u = fmaf(dDy, gyc, fmaf(dDx, gxc, Dc * (uxx + uyy))) * k2;
+    }
     float dv, dw, jion;
     reaction(c, vv[k], ww[k], uc, dv, dw, jion);
     if (STIM) {
@@ -198,14 +226,15 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
   }
 }
 
-// Four waves per block, one (strip, band) tile each, band-major within a
-// strip so that consecutive tiles share row halos.
-template <int R, bool EULER>
-__global__ void __launch_bounds__(256)
-k_stream(Coef c, const float* __restrict__ D, const float* __restrict__ v,
-         const float* __restrict__ w, const float* __restrict__ u, int H, int W, StimArgs sa,
-         float dt, float inv_dx2, float* __restrict__ o_v, float* __restrict__ o_w,
-         float* __restrict__ o_u, StreamGeom g) {
+// One wave per (strip, band) tile, band-major within a strip so that
+// consecutive tiles share row halos; four waves per block. PC: Euler steps
+// from precomputed coefficients (k_stream_c), else derived from D.
+template <int R, bool EULER, bool PC>
+__device__ __forceinline__ void stream_kernel_body(
+    const Coef& c, const float* __restrict__ D, const float4* __restrict__ cf,
+    const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ u,
+    int H, int W, const StimArgs& sa, float dt, float inv_dx2, float* __restrict__ o_v,
+    float* __restrict__ o_w, float* __restrict__ o_u, const StreamGeom& g) {
   const int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x >> 6));
   if (tile >= g.n_tiles) return;  // whole wave leaves; the kernel has no barrier
   const int strip = tile / g.n_bands, band = tile - strip * g.n_bands;
@@ -215,9 +244,9 @@ k_stream(Coef c, const float* __restrict__ D, const float* __restrict__ v,
   const bool yg = cs < 3 || cs + kStripOut + 3 > W;  // strip within 3 columns
   const int sel = (xg ? 1 : 0) | (yg ? 2 : 0) | (sa.n ? 4 : 0);
   const StimListView sv{sa};
-#define FK_TILE(X, Y, S)                                                                   \
-  stream_tile<R, EULER, X, Y, S>(c, D, v, w, u, H, W, sv, dt, inv_dx2, o_v, o_w, o_u, rs, \
-                                 cs)
+#define FK_TILE(X, Y, S)                                                                  \
+  stream_tile<R, EULER, X, Y, S, StimListView, PC>(c, D, cf, v, w, u, H, W, sv, dt, inv_dx2, \
+                                                   o_v, o_w, o_u, rs, cs)
   switch (sel) {
     case 0: FK_TILE(false, false, false); break;
     case 1: FK_TILE(true, false, false); break;
@@ -229,6 +258,26 @@ k_stream(Coef c, const float* __restrict__ D, const float* __restrict__ v,
     default: FK_TILE(true, true, true); break;
   }
 #undef FK_TILE
+}
+
+template <int R, bool EULER>
+__global__ void __launch_bounds__(256)
+k_stream(Coef c, const float* __restrict__ D, const float* __restrict__ v,
+         const float* __restrict__ w, const float* __restrict__ u, int H, int W, StimArgs sa,
+         float dt, float inv_dx2, float* __restrict__ o_v, float* __restrict__ o_w,
+         float* __restrict__ o_u, StreamGeom g) {
+  stream_kernel_body<R, EULER, false>(c, D, nullptr, v, w, u, H, W, sa, dt, inv_dx2, o_v, o_w,
+                                      o_u, g);
+}
+
+template <int R>
+__global__ void __launch_bounds__(256)
+k_stream_c(Coef c, const float4* __restrict__ cf, const float* __restrict__ v,
+           const float* __restrict__ w, const float* __restrict__ u, int H, int W, StimArgs sa,
+           float dt, float* __restrict__ o_v, float* __restrict__ o_w, float* __restrict__ o_u,
+           StreamGeom g) {
+  stream_kernel_body<R, true, true>(c, nullptr, cf, v, w, u, H, W, sa, dt, 0.0f, o_v, o_w, o_u,
+                                    g);
 }
 
 }  // namespace fk

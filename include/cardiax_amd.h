@@ -63,13 +63,16 @@ const char* fk_last_error(void);
 int32_t fk_abi_version(void);
 
 /* Kernel used for the forward step by fk_step, fk_step_euler and
- * fk_forward_euler: 0 = automatic (default), 1 = plain one thread per cell,
- * 2 / 3 = LDS tile of 16 / 32 rows, 4 / 5 / 6 = streaming wavefront bands of
- * 8 / 16 / 32 rows, 7 / 8 / 9 / 10 = temporal blocking on 64x64 tiles (2 / 4
- * steps per launch) and 32x32 tiles (2 / 4 steps per launch; fk_step and
- * fk_step_euler use variant 4 then). Process-wide; set by the autotuner. No reference
- * counterpart (the reference leaves kernel choice to XLA). Returns the
- * previous setting, or FK_ERR_ARG for an unknown variant. */
+ * fk_forward_euler: 0 = automatic (default: the autotuned choice for the
+ * grid shape, else 4), 1 = plain one thread per cell, 2 / 3 = LDS tile of 16 /
+ * 32 rows, 4 / 5 / 6 = streaming wavefront bands of 8 / 16 / 32 rows, 7 / 8 /
+ * 9 / 10 = temporal blocking on 64x64 tiles (2 / 4 steps per launch) and 32x32
+ * tiles (2 / 4 steps per launch), 11 / 12 = streaming bands of 8 / 16 rows
+ * with the static diffusion coefficients precomputed once per
+ * fk_forward_euler call. fk_step and fk_step_euler run variant 4 in place of
+ * 7-11 and 5 in place of 12. Process-wide. No reference counterpart (the
+ * reference leaves kernel choice to XLA). Returns the previous setting, or
+ * FK_ERR_ARG for an unknown variant. */
 int32_t fk_set_forward_variant(int32_t variant);
 
 /* Autotuner of the forward step kernel. No reference counterpart (the
@@ -107,7 +110,8 @@ int32_t fk_step_euler(const fk_params* params, const float* D, const float* v,
                       const fk_stimulus* stimuli, int32_t n_stimuli, double t, float dt,
                       float dx, float* v_out, float* w_out, float* u_out, void* stream);
 
-/* Scratch bytes fk_forward_euler needs for an H x W grid. */
+/* Scratch bytes fk_forward_euler needs for an H x W grid (a ping-pong state
+ * and the precomputed diffusion coefficients). */
 size_t fk_forward_workspace_bytes(int32_t H, int32_t W);
 
 /* cardiax/solve.py:92-100 -- the Euler loop over i = t, t+1, ... while

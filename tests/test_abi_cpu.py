@@ -38,7 +38,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_workspace_query():
     L = _lib.lib()
-    assert L.fk_forward_workspace_bytes(128, 64) == 3 * 128 * 64 * 4
+    # ping-pong state (3 fields, 256-byte aligned) + one float4 of coefficients per cell
+    assert L.fk_forward_workspace_bytes(128, 64) == 3 * 128 * 64 * 4 + 16 * 128 * 64
+    assert L.fk_forward_workspace_bytes(5, 7) == 512 + 16 * 35
     assert L.fk_forward_workspace_bytes(0, 5) == 0
 
 

@@ -19,6 +19,18 @@ from test_gpu_batch import _sims
 pytestmark = pytest.mark.gpu
 
 
+def _per_sim(sims, n, rows):
+    """Per-simulation runs on the single-grid kernel with the same band height
+    (bitwise comparisons need the same instantiation of the step code)."""
+    L = _lib.lib()
+    prev = L.fk_set_forward_variant(_lib.VARIANTS["stream%d" % rows])
+    try:
+        return [solve._forward_euler(s.state, s.t, s.t + n, s.params, s.diffusivity,
+                                     s.stimuli, s.dt, s.dx) for s in sims]
+    finally:
+        L.fk_set_forward_variant(prev)
+
+
 def test_forward_autotune_records_and_is_used(gpu):
     shape = (96, 130)
     autotune.reset()
@@ -26,8 +38,8 @@ def test_forward_autotune_records_and_is_used(gpu):
         assert autotune.lookup(shape) == "auto"
         res = autotune.tune(shape, n_steps=20)
         timed = set(res["times_ms"])
-        assert timed == {"stream8", "stream16", "stream32", "tile16", "tile32", "tb64x2",
-                         "tb64x4", "tb32x2", "tb32x4"}
+        assert timed == {"stream8", "stream16", "stream32", "streamc8", "streamc16", "tile16",
+                         "tile32", "tb64x2", "tb64x4", "tb32x2", "tb32x4"}
         assert all(t > 0 for t in res["times_ms"].values())
         assert res["best"] == min(res["times_ms"], key=res["times_ms"].get)
         assert autotune.lookup(shape) == res["best"]
@@ -63,9 +75,7 @@ def test_batch_autotune_keeps_results_bitwise(gpu):
             assert torch.equal(x, torch.stack(list(s)))
         b.run(n)
         torch.cuda.synchronize()
-        for s, got in zip(sims, b.states):
-            ref = solve._forward_euler(s.state, s.t, s.t + n, s.params, s.diffusivity,
-                                       s.stimuli, s.dt, s.dx)
+        for got, ref in zip(b.states, _per_sim(sims, n, res["rows"])):
             for a, r in zip(got, ref):
                 assert torch.equal(a, r)
 
@@ -79,9 +89,7 @@ def test_batch_band_heights_match_per_simulation_runs(gpu, rows, use_graph):
         b.set_rows(rows)
         b.run(n, use_graph=use_graph)
         torch.cuda.synchronize()
-        for s, got in zip(sims, b.states):
-            ref = solve._forward_euler(s.state, s.t, s.t + n, s.params, s.diffusivity,
-                                       s.stimuli, s.dt, s.dx)
+        for got, ref in zip(b.states, _per_sim(sims, n, rows)):
             for a, r in zip(got, ref):
                 assert torch.equal(a, r)
         with pytest.raises(ValueError):
