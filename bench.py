@@ -197,6 +197,34 @@ def bench_adjoint(seed, size=256, n_steps=2000, frame_every=50, reps=2):
             "grad_finite": bool(torch.isfinite(D.grad).all().item())}
 
 
+def bench_inverse(seed, n=32, iters=50, n_steps=1000, frame_every=50):
+    """Batched inverse problem: n instances of 128^2..512^2 each recover a
+    hidden scar D-map by `iters` Adam iterations of forward + adjoint."""
+    import numpy as np
+
+    from cardiax_amd import inverse
+
+    inst = inverse.make_instances(seed, n=n, n_steps=n_steps, frame_every=frame_every)
+    inverse.recover(inst[:2], iters=1)  # warm-up (kernels, allocator)
+    tf = inverse.forward_only_seconds(inst)
+    res = inverse.recover(inst, iters=iters)
+    cells = sum(int(x.D_true.numel()) for x in inst)
+    loss = res["loss"]
+    return {"workload": "%d instances, sides %s, %d Adam iterations of forward + adjoint, "
+                        "%d Euler steps each, frames every %d, float64 loss"
+                        % (n, sorted({int(x.D_true.shape[0]) for x in inst}), iters, n_steps,
+                           frame_every),
+            "seconds": round(res["seconds"], 3),
+            "fwd_adjoint_gcups": round(cells * n_steps * iters / res["seconds"] / 1e9, 3),
+            "cost_ratio": round(res["seconds"] / iters / tf, 2),
+            # summed over instances (a few see no wave reach their scars, so their
+            # loss starts near 1e-17 and a per-instance ratio says nothing)
+            "loss_sum_final_over_initial": round(float(loss[:, -1].sum() / loss[:, 0].sum()), 4),
+            "loss_ratio_median": round(float(np.median(loss[:, -1] / loss[:, 0])), 4),
+            "d_err_initial": round(float(np.mean(res["d_err_initial"])), 4),
+            "d_err_final": round(float(np.mean(res["d_err_final"])), 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +243,8 @@ def main():
                     help="simulations of the batched workload (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--inverse-iters", type=int, default=50,
+                    help="Adam iterations of the batched inverse problem (0 = skip it)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (the default kernel, stream8, runs)")
     args = ap.parse_args()
@@ -316,6 +346,11 @@ def main():
         log("adjoint workload ...")
         extra["adjoint"] = bench_adjoint(args.seed)
         log("adjoint: %s" % json.dumps(extra["adjoint"]))
+
+    if args.inverse_iters > 0:
+        log("inverse problem ...")
+        extra["inverse"] = bench_inverse(args.seed, iters=args.inverse_iters)
+        log("inverse: %s" % json.dumps(extra["inverse"]))
 
     cpu = None
     if not args.no_cpu:

@@ -36,7 +36,10 @@ class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, D, tau_d, tau_r, tau_si, cfg):
         dev = D.device
-        prm = _with_taus(cfg["params"], tau_d.item(), tau_r.item(), tau_si.item())
+        # host values of the taus: given floats need no device round trip
+        hv = cfg["tau_host"]
+        prm = _with_taus(cfg["params"], *(h if h is not None else t.item()
+                                          for h, t in zip(hv, (tau_d, tau_r, tau_si))))
         st = State(*(on_device(x, dev).contiguous() for x in cfg["state0"]))
         H, W = st.u.shape
         n, fe, t0 = cfg["n_steps"], cfg["frame_every"], cfg["t0"]
@@ -122,19 +125,25 @@ def rollout(D: torch.Tensor, tau_d, tau_r, tau_si, state0, params, stimuli: Sequ
     """u after every ``frame_every`` Euler steps of cardiax/solve.py:92-100,
     shape (n_steps // frame_every, H, W), differentiable with respect to D
     (H x W) and the 0-d tensors tau_d, tau_r, tau_si (which replace those
-    fields of ``params``). Other fields of ``params``, the initial state and
-    the stimuli are constants."""
+    fields of ``params``; pass floats for constants, which also spares the
+    host a device synchronisation). Other fields of ``params``, the initial
+    state and the stimuli are constants."""
     dev = require_gpu("rollout")
     if int(frame_every) <= 0 or int(n_steps) < 0:
         raise ValueError("frame_every must be positive and n_steps non-negative")
     D = on_device(D, dev) if not isinstance(D, torch.Tensor) else D
     if D.dtype != torch.float32 or D.device != dev:
         raise ValueError("D must be a float32 tensor on the GPU")
-    taus = []
+    taus, host = [], []
     for x in (tau_d, tau_r, tau_si):
-        t = x if isinstance(x, torch.Tensor) else torch.tensor(float(x))
-        taus.append(t.to(device=dev, dtype=torch.float32).reshape(()))
+        if isinstance(x, torch.Tensor):
+            taus.append(x.to(device=dev, dtype=torch.float32).reshape(()))
+            host.append(None)
+        else:  # a constant: no gradient, value known on the host
+            taus.append(torch.full((), float(x), dtype=torch.float32, device=dev))  # no sync
+            host.append(float(x))
     cfg = {"params": params, "state0": state0, "stimuli": list(stimuli or []),
            "t0": _scalar(t0), "n_steps": int(n_steps), "frame_every": int(frame_every),
-           "dt": float(dt), "dx": float(dx), "checkpoint_every": checkpoint_every}
+           "dt": float(dt), "dx": float(dx), "checkpoint_every": checkpoint_every,
+           "tau_host": host}
     return _Rollout.apply(D, taus[0], taus[1], taus[2], cfg)

@@ -1,0 +1,161 @@
+"""Batched inverse problem: recover scar D-maps from sampled u frames.
+
+Each instance holds a hidden diffusivity map D* and the u frames that the
+forward-Euler solve (cardiax/solve.py:92-100) produces from it. Starting
+from the healthy map D = 1e-3, Adam minimises the frame MSE (accumulated in
+float64) by gradients from the hand-written adjoint (``adjoint.rollout``),
+keeping D inside the reference's range [1e-4, 1e-3] (deepx/generate.py:80).
+
+D is parameterised by a coarse grid (one node per ``coarse`` cells) and its
+bilinear interpolation: Adam's first steps move every free value by about
+the learning rate whatever the gradient's size, and a cell-by-cell D of such
+steps has the checkerboard structure that the reference's explicit scheme
+with its non-conservative D_x u_x term does not damp. The interpolation is
+two small matrix products (deterministic, unlike scattered atomics).
+
+The reference has no such driver (its deepx package learns the inverse map
+with a network instead). The instances run interleaved on several HIP streams
+so that small grids, which leave most of the GPU idle, overlap; no step of
+the loop synchronises the host with the device.
+"""
+from __future__ import annotations
+
+import time
+from typing import List, NamedTuple, Sequence
+
+import numpy as np
+import torch
+
+from . import adjoint, solve
+from . import workloads as wl
+from ._device import require_gpu
+from .params import PARAMSET_3
+from .stimulus import Direction, Protocol, _stripe_mask
+
+D_MIN, D_MAX = 1e-4, 1e-3
+
+
+class Instance(NamedTuple):
+    """One recovery problem: the hidden map, its target frames and inputs."""
+    D_true: torch.Tensor
+    target: torch.Tensor      # (n_frames, H, W)
+    state0: solve.State
+    params: object
+    stimuli: list
+    n_steps: int
+    frame_every: int
+
+
+def make_instances(seed: int = 0, n: int = 32, sides=(128, 256, 384, 512),
+                   n_steps: int = 1000, frame_every: int = 50, dt: float = wl.DT,
+                   dx: float = wl.DX) -> List[Instance]:
+    """``n`` square instances with sides drawn from ``sides``: a seeded scar
+    map, S1 (WEST stripe at step 0) and S2 (NORTH half at a seeded step in
+    the first half of the run), and the target frames of the hidden map."""
+    dev = require_gpu("inverse")
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        side = int(rng.choice(sides))
+        shape = (side, side)
+        Dt = torch.from_numpy(wl.scar_map(shape, rng)).to(dev)
+        f1 = torch.from_numpy(_stripe_mask(shape, Direction.WEST, 0.1, wl.MODULUS)).to(dev)
+        f2 = torch.from_numpy(_stripe_mask(shape, Direction.NORTH, 0.5, wl.MODULUS)).to(dev)
+        s2 = int(rng.integers(n_steps // 5, n_steps // 2))
+        stims = [(Protocol(0, 2, wl.PERIOD), f1), (Protocol(s2, 2, wl.PERIOD), f2)]
+        st = x = solve.init(shape)
+        frames = []
+        for t in range(0, n_steps - frame_every + 1, frame_every):
+            x = solve._forward_euler(x, t, t + frame_every, PARAMSET_3, Dt, stims, dt, dx)
+            frames.append(x.u)
+        out.append(Instance(Dt, torch.stack(frames), st, PARAMSET_3, stims, n_steps,
+                            frame_every))
+    return out
+
+
+def interp_matrix(n: int, m: int, device) -> torch.Tensor:
+    """(n, m) matrix of 1-D linear interpolation from m nodes spanning [0, n-1]
+    (end points aligned) to the n cell centres."""
+    x = torch.arange(n, dtype=torch.float64) * ((m - 1) / max(n - 1, 1))
+    lo = torch.clamp(x.floor().long(), 0, m - 2)
+    f = x - lo
+    M = torch.zeros((n, m), dtype=torch.float64)
+    M[torch.arange(n), lo] = 1.0 - f
+    M[torch.arange(n), lo + 1] += f
+    return M.to(device=device, dtype=torch.float32)
+
+
+class _Param(NamedTuple):
+    theta: torch.Tensor  # coarse nodes (requires grad)
+    Ry: torch.Tensor
+    Rx: torch.Tensor
+
+    def D(self) -> torch.Tensor:
+        return self.Ry @ self.theta @ self.Rx.t()
+
+
+def recover(instances: Sequence[Instance], iters: int = 50, lr: float = 2e-5,
+            n_streams: int = 8, coarse: int = 8, dt: float = wl.DT,
+            dx: float = wl.DX) -> dict:
+    """Runs ``iters`` Adam iterations of forward + adjoint on every instance.
+    Returns the estimates and per-instance losses and D errors (relative L2
+    distance to the hidden map, before and after), plus the wall time."""
+    dev = require_gpu("inverse")
+    streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
+    params, opts = [], []
+    for inst in instances:
+        H, W = inst.D_true.shape
+        h, w = max(2, -(-H // coarse) + 1), max(2, -(-W // coarse) + 1)
+        th = torch.full((h, w), D_MAX, dtype=torch.float32, device=dev, requires_grad=True)
+        params.append(_Param(th, interp_matrix(H, h, dev), interp_matrix(W, w, dev)))
+        opts.append(torch.optim.Adam([th], lr=lr))
+    losses = torch.zeros((len(instances), iters), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(iters):
+        for k, (inst, prm, opt) in enumerate(zip(instances, params, opts)):
+            s = streams[k % len(streams)]
+            with torch.cuda.stream(s):
+                opt.zero_grad(set_to_none=True)
+                p = inst.params
+                D = prm.D().contiguous()
+                fr = adjoint.rollout(D, p.tau_d, p.tau_r, p.tau_si, inst.state0, p, inst.stimuli,
+                                     0, inst.n_steps, inst.frame_every, dt, dx)
+                diff = (fr - inst.target).double()
+                loss = (diff * diff).mean()  # float64 accumulation
+                loss.backward()
+                opt.step()
+                with torch.no_grad():  # interpolation keeps D within the nodes' range
+                    prm.theta.clamp_(D_MIN, D_MAX)
+                losses[k, it] = loss.detach()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    err0, err1, Ds = [], [], []
+    for inst, prm in zip(instances, params):
+        with torch.no_grad():
+            D = prm.D()
+        Ds.append(D)
+        ref = inst.D_true.double()
+        nrm = float(torch.linalg.norm(ref))
+        err0.append(float(torch.linalg.norm(torch.full_like(ref, D_MAX) - ref)) / nrm)
+        err1.append(float(torch.linalg.norm(D.double() - ref)) / nrm)
+    return {"D": Ds, "loss": losses.cpu().numpy(), "seconds": wall,
+            "d_err_initial": err0, "d_err_final": err1}
+
+
+def forward_only_seconds(instances: Sequence[Instance], n_streams: int = 8,
+                         dt: float = wl.DT, dx: float = wl.DX) -> float:
+    """Wall time of one forward rollout of every instance (no gradient), the
+    denominator of the forward + adjoint cost ratio."""
+    streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        for k, inst in enumerate(instances):
+            with torch.cuda.stream(streams[k % len(streams)]):
+                p = inst.params
+                adjoint.rollout(torch.full_like(inst.D_true, D_MAX), p.tau_d, p.tau_r, p.tau_si,
+                                inst.state0, p, inst.stimuli, 0, inst.n_steps,
+                                inst.frame_every, dt, dx)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
