@@ -398,13 +398,18 @@ __global__ void k_adj_zero(float* a, size_t n) {
     a[i] = 0.0f;
 }
 
+constexpr int kRedBlocks = 512;
+
 // Sums of the three per-cell tau accumulators (x, y, z of n float4), in
-// double.
-__global__ void k_adj_reduce(const float4* __restrict__ part, size_t n, float* g_tau) {
+// double: each block reduces a grid-stride slice into part[block][0..2]
+// (k_adj_reduce), then one block adds the block sums (k_adj_reduce_final).
+__global__ void __launch_bounds__(256)
+k_adj_reduce(const float4* __restrict__ acc, size_t n, double* __restrict__ part) {
   __shared__ double s[3][256];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (size_t b = threadIdx.x; b < n; b += blockDim.x) {
-    const float4 p = part[b];
+  for (size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x; b < n;
+       b += (size_t)gridDim.x * blockDim.x) {
+    const float4 p = acc[b];
     a0 += (double)p.x;
     a1 += (double)p.y;
     a2 += (double)p.z;
@@ -419,13 +424,32 @@ __global__ void k_adj_reduce(const float4* __restrict__ part, size_t n, float* g
     __syncthreads();
   }
   if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[3 * (size_t)blockIdx.x + k] = s[k][0];
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_reduce_final(const double* __restrict__ part, int n_part, float* g_tau) {
+  __shared__ double s[3][256];
+  for (int k = 0; k < 3; ++k) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < n_part; b += blockDim.x) a += part[3 * b + k];
+    s[k][threadIdx.x] = a;
+  }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
     for (int k = 0; k < 3; ++k) g_tau[k] = (float)s[k][0];
 }
 
 // Workspace layout: [tables (2 axes)][Dx][Dy][tau accumulators: float4 per cell][cotangent
 // ping-pong: 3 fields][derived fields: 2 sets of 7 (8 floats per cell)]
+// [tau block sums]
 struct AdjLayout {
-  size_t tab, dx, dy, part, lam, pre, total;
+  size_t tab, dx, dy, part, lam, pre, red, total;
 };
 
 static AdjLayout adj_layout(int H, int W) {
@@ -443,6 +467,7 @@ static AdjLayout adj_layout(int H, int W) {
   L.part = take(cells * sizeof(float4));
   L.lam = take(3 * cells * sizeof(float));
   L.pre = take(2 * 8 * cells * sizeof(float));
+  L.red = take(3 * kRedBlocks * sizeof(double));
   L.total = o;
   return L;
 }
@@ -601,13 +626,16 @@ int32_t fk_adjoint_segment(const fk_params* params, const float* D, int32_t H, i
   return FK_OK;
 }
 
-int32_t fk_adjoint_tau_grads(const void* work, int32_t H, int32_t W, float* g_tau,
-                             void* stream) {
+int32_t fk_adjoint_tau_grads(void* work, int32_t H, int32_t W, float* g_tau, void* stream) {
   ErrorScope es;
   if (!check_grid(H, W) || !check_ptrs({work, g_tau})) return last_code();
   const AdjLayout L = adj_layout(H, W);
-  k_adj_reduce<<<1, 256, 0, (hipStream_t)stream>>>(
-      (const float4*)((const char*)work + L.part), (size_t)H * W, g_tau);
+  const size_t n = (size_t)H * W;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, (size_t)kRedBlocks);
+  double* part = (double*)((char*)work + L.red);
+  k_adj_reduce<<<blocks, 256, 0, (hipStream_t)stream>>>(
+      (const float4*)((char*)work + L.part), n, part);
+  k_adj_reduce_final<<<1, 256, 0, (hipStream_t)stream>>>(part, blocks, g_tau);
   HIP_CHECK(hipGetLastError());
   return FK_OK;
 }

@@ -202,9 +202,9 @@ int32_t fk_adjoint_segment(const fk_params* params, const float* D, int32_t H, i
                            void* stream);
 
 /* Writes the accumulated gradients with respect to (tau_d, tau_r, tau_si) to
- * g_tau (3 floats, device memory). */
-int32_t fk_adjoint_tau_grads(const void* work, int32_t H, int32_t W, float* g_tau,
-                             void* stream);
+ * g_tau (3 floats, device memory). Uses a scratch area of the workspace for
+ * block sums; the accumulators themselves are left as they are. */
+int32_t fk_adjoint_tau_grads(void* work, int32_t H, int32_t W, float* g_tau, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Batched forward Euler. No reference counterpart: the reference runs one
