@@ -30,7 +30,9 @@ KERNELS = {"stream8": ("k_stream<8, true>", 1), "stream16": ("k_stream<16, true>
            "tile32": ("k_tile<32, true>", 1), "tb64x2": ("k_tb<64, 2, 1024>", 2),
            "tb64x4": ("k_tb<64, 4, 1024>", 4), "tb32x2": ("k_tb<32, 2, 256>", 2),
            "tb32x4": ("k_tb<32, 4, 512>", 4), "streamc8": ("k_stream_c<8>", 1),
-           "streamc16": ("k_stream_c<16>", 1)}
+           "streamc16": ("k_stream_c<16>", 1), "stream4": ("k_stream<4, true>", 1),
+           "streamc4": ("k_stream_c<4>", 1), "stream2": ("k_stream<2, true>", 1),
+           "streamc2": ("k_stream_c<2>", 1)}
 
 
 def log(*a):

@@ -26,7 +26,8 @@ FK_MAX_STIMULI = 32
 # Forward kernel variants (fk_set_forward_variant).
 VARIANTS = {"auto": 0, "plain": 1, "tile16": 2, "tile32": 3, "stream8": 4, "stream16": 5,
             "stream32": 6, "tb64x2": 7, "tb64x4": 8, "tb32x2": 9, "tb32x4": 10,
-            "streamc8": 11, "streamc16": 12}
+            "streamc8": 11, "streamc16": 12, "stream4": 13, "streamc4": 14,
+            "stream2": 15, "streamc2": 16}
 
 c_f = ctypes.c_float
 c_d = ctypes.c_double

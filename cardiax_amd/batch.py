@@ -95,19 +95,19 @@ class Batch:
         return int(_lib.lib().fk_batch_n_items(self._h))
 
     def autotune(self, n_steps: int = 64) -> dict:
-        """Time the batched step with 8- and 16-row bands and keep the faster
-        for later runs (fk_batch_autotune); the simulations' states are not
-        touched. Returns ``{"rows": R, "times_ms": [t8, t16]}``."""
+        """Time the batched step with 2-, 4-, 8- and 16-row bands and keep the
+        fastest for later runs (fk_batch_autotune); the simulations' states are
+        not touched. Returns ``{"rows": R, "times_ms": [t2, t4, t8, t16]}``."""
         if self._h is None:
             raise RuntimeError("batch is closed")
-        times = (ctypes.c_float * 2)()
+        times = (ctypes.c_float * 4)()
         rows = ctypes.c_int32(0)
         _lib.check(_lib.lib().fk_batch_autotune(self._h, int(n_steps), stream_handle(), times,
                                             ctypes.byref(rows)))
         return {"rows": int(rows.value), "times_ms": [round(float(t), 4) for t in times]}
 
     def set_rows(self, rows: int) -> None:
-        """Pin the band height (8 or 16) of later runs (fk_batch_set_rows)."""
+        """Pin the band height (2, 4, 8 or 16) of later runs (fk_batch_set_rows)."""
         if self._h is None:
             raise RuntimeError("batch is closed")
         _lib.check(_lib.lib().fk_batch_set_rows(self._h, int(rows)))

@@ -28,7 +28,9 @@
 namespace fk {
 
 // Rows per wave band: one work list per option, chosen by fk_batch_autotune.
-constexpr int kBatchRows[2] = {8, 16};
+constexpr int kNumRowOpts = 4;
+constexpr int kBatchRows[kNumRowOpts] = {2, 4, 8, 16};
+constexpr int kDefaultRowOpt = 2;  // 8 rows
 constexpr int kGraphSteps = 32;   // Euler steps recorded in one graph (even)
 
 struct BatchStim {
@@ -167,14 +169,14 @@ using namespace fk;
 // Opaque batch handle.
 struct fk_batch {
   int n_sims = 0;
-  int opt = 0;            // index into kBatchRows
-  int n_items_opt[2] = {0, 0};
+  int opt = kDefaultRowOpt;  // index into kBatchRows
+  int n_items_opt[kNumRowOpts] = {};
   std::vector<fk_sim> host_sims;   // caller's descriptors (state pointers)
   std::vector<size_t> offset;      // per-sim cell offset in the pools
   size_t total_cells = 0;
   char* dev = nullptr;             // one allocation: descriptors, items, counter, pools
   SimDesc* d_sims = nullptr;
-  TileItem* d_items[2] = {nullptr, nullptr};
+  TileItem* d_items[kNumRowOpts] = {};
   int64_t* d_step = nullptr;
   uint32_t* d_mask = nullptr;  // per-simulation stimulus activity of the current step
   float* pool[2] = {nullptr, nullptr};  // ping-pong pools of 3 * total_cells floats
@@ -218,16 +220,18 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out) {
   fk_batch* b = new fk_batch();
   b->n_sims = n;
   b->host_sims.assign(sims, sims + n);
-  std::vector<TileItem> items[2];
+  std::vector<TileItem> items[kNumRowOpts];
   b->offset.resize(n);
   for (int i = 0; i < n; ++i) {
     const fk_sim& s = sims[i];
     b->offset[i] = b->total_cells;
     b->total_cells += (size_t)s.H * s.W;
-    for (int o = 0; o < 2; ++o) {
+    for (int o = 0; o < kNumRowOpts; ++o) {
       const int R = kBatchRows[o];
-      const StreamGeom g = o == 0 ? make_stream_geom<kBatchRows[0]>(s.H, s.W)
-                                  : make_stream_geom<kBatchRows[1]>(s.H, s.W);
+      const StreamGeom g = o == 0   ? make_stream_geom<kBatchRows[0]>(s.H, s.W)
+                           : o == 1 ? make_stream_geom<kBatchRows[1]>(s.H, s.W)
+                           : o == 2 ? make_stream_geom<kBatchRows[2]>(s.H, s.W)
+                                    : make_stream_geom<kBatchRows[3]>(s.H, s.W);
       for (int strip = 0; strip < g.n_strips; ++strip)
         for (int band = 0; band < g.n_bands; ++band) {
           TileItem t;
@@ -241,12 +245,13 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out) {
         }
     }
   }
-  b->n_items_opt[0] = (int)items[0].size();
-  b->n_items_opt[1] = (int)items[1].size();
+  size_t sz_it[kNumRowOpts], sz_items = 0;
+  for (int o = 0; o < kNumRowOpts; ++o) {
+    b->n_items_opt[o] = (int)items[o].size();
+    sz_it[o] = ((sizeof(TileItem) * items[o].size() + 255) / 256) * 256;
+    sz_items += sz_it[o];
+  }
   const size_t sz_sims = ((sizeof(SimDesc) * n + 255) / 256) * 256;
-  const size_t sz_it0 = ((sizeof(TileItem) * items[0].size() + 255) / 256) * 256;
-  const size_t sz_it1 = ((sizeof(TileItem) * items[1].size() + 255) / 256) * 256;
-  const size_t sz_items = sz_it0 + sz_it1;
   const size_t sz_pool = 3 * b->total_cells * sizeof(float);
   const size_t sz_mask = ((sizeof(uint32_t) * n + 255) / 256) * 256;
   const size_t total = sz_sims + sz_items + 256 + sz_mask + 2 * sz_pool;
@@ -256,8 +261,13 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out) {
     return set_error(FK_ERR_HIP, "hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
   }
   b->d_sims = (SimDesc*)b->dev;
-  b->d_items[0] = (TileItem*)(b->dev + sz_sims);
-  b->d_items[1] = (TileItem*)(b->dev + sz_sims + sz_it0);
+  {
+    size_t at = sz_sims;
+    for (int o = 0; o < kNumRowOpts; ++o) {
+      b->d_items[o] = (TileItem*)(b->dev + at);
+      at += sz_it[o];
+    }
+  }
   b->d_step = (int64_t*)(b->dev + sz_sims + sz_items);
   b->d_mask = (uint32_t*)(b->dev + sz_sims + sz_items + 256);
   b->pool[0] = (float*)(b->dev + sz_sims + sz_items + 256 + sz_mask);
@@ -283,7 +293,7 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out) {
                    s.stimuli[k].field};
   }
   e = hipMemcpy(b->d_sims, desc.data(), sizeof(SimDesc) * n, hipMemcpyHostToDevice);
-  for (int o = 0; o < 2 && e == hipSuccess; ++o)
+  for (int o = 0; o < kNumRowOpts && e == hipSuccess; ++o)
     e = hipMemcpy(b->d_items[o], items[o].data(), sizeof(TileItem) * items[o].size(),
                   hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
@@ -310,12 +320,13 @@ static hipError_t batch_step(fk_batch* b, int rel, int parity, hipStream_t s) {
                                                         b->d_mask);
   const int n_items = b->n_items_opt[b->opt];
   const int blocks = (n_items + 3) / 4;
-  if (b->opt == 0)
-    k_batch_step<kBatchRows[0]><<<blocks, 256, 0, s>>>(b->d_sims, b->d_items[0], n_items,
-                                                       b->d_mask, parity);
-  else
-    k_batch_step<kBatchRows[1]><<<blocks, 256, 0, s>>>(b->d_sims, b->d_items[1], n_items,
-                                                       b->d_mask, parity);
+  const TileItem* it = b->d_items[b->opt];
+  switch (b->opt) {
+    case 0: k_batch_step<kBatchRows[0]><<<blocks, 256, 0, s>>>(b->d_sims, it, n_items, b->d_mask, parity); break;
+    case 1: k_batch_step<kBatchRows[1]><<<blocks, 256, 0, s>>>(b->d_sims, it, n_items, b->d_mask, parity); break;
+    case 2: k_batch_step<kBatchRows[2]><<<blocks, 256, 0, s>>>(b->d_sims, it, n_items, b->d_mask, parity); break;
+    default: k_batch_step<kBatchRows[3]><<<blocks, 256, 0, s>>>(b->d_sims, it, n_items, b->d_mask, parity); break;
+  }
   return hipGetLastError();
 }
 
@@ -351,8 +362,8 @@ int32_t fk_batch_autotune(fk_batch* b, int32_t n_steps, void* stream, float* tim
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  float ms[2] = {0.0f, 0.0f};
-  for (int o = 0; o < 2 && e == hipSuccess; ++o) {
+  float ms[kNumRowOpts] = {};
+  for (int o = 0; o < kNumRowOpts && e == hipSuccess; ++o) {
     b->opt = o;
     k_batch_set<<<1, 64, 0, s>>>(b->d_step, 0);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) e = batch_step(b, k, k & 1, s);
@@ -364,7 +375,11 @@ int32_t fk_batch_autotune(fk_batch* b, int32_t n_steps, void* stream, float* tim
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  b->opt = (e == hipSuccess && ms[1] < ms[0]) ? 1 : 0;
+  int best = kDefaultRowOpt;
+  if (e == hipSuccess)
+    for (int o = 0; o < kNumRowOpts; ++o)
+      if (ms[o] < ms[best]) best = o;
+  b->opt = best;
   // graphs recorded for another work list are stale
   for (auto& x : b->exec)
     if (x) {
@@ -373,10 +388,8 @@ int32_t fk_batch_autotune(fk_batch* b, int32_t n_steps, void* stream, float* tim
     }
   if (e != hipSuccess) return set_error(FK_ERR_HIP, "batch autotune failed: %s",
                                         hipGetErrorString(e));
-  if (times_ms) {
-    times_ms[0] = ms[0];
-    times_ms[1] = ms[1];
-  }
+  if (times_ms)
+    for (int o = 0; o < kNumRowOpts; ++o) times_ms[o] = ms[o];
   if (best_rows) *best_rows = kBatchRows[b->opt];
   return FK_OK;
 }
@@ -385,9 +398,9 @@ int32_t fk_batch_set_rows(fk_batch* b, int32_t rows) {
   ErrorScope es;
   if (b == nullptr) return set_error(FK_ERR_ARG, "batch is null");
   int o = -1;
-  for (int k = 0; k < 2; ++k)
+  for (int k = 0; k < kNumRowOpts; ++k)
     if (rows == kBatchRows[k]) o = k;
-  if (o < 0) return set_error(FK_ERR_ARG, "band height must be 8 or 16, got %d", rows);
+  if (o < 0) return set_error(FK_ERR_ARG, "band height must be 2, 4, 8 or 16, got %d", rows);
   if (o != b->opt) {
     HIP_CHECK(hipDeviceSynchronize());  // no queued launch may still use the old list
     for (auto& x : b->exec)

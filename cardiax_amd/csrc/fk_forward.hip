@@ -290,10 +290,23 @@ enum Variant : int {
   kTb32x4 = 10,  // 32 x 32 tiles, 4 steps per launch
   kStreamC8 = 11,   // streaming bands of 8 / 16 rows with the static diffusion
   kStreamC16 = 12,  // coefficients precomputed once per solve
-  kNumVariants = 13
+  kStream4 = 13,    // streaming bands of 4 rows
+  kStreamC4 = 14,   // ... with precomputed coefficients
+  kStream2 = 15,    // streaming bands of 2 rows
+  kStreamC2 = 16,   // ... with precomputed coefficients
+  kNumVariants = 17
 };
 
-static bool is_pc(int v) { return v == kStreamC8 || v == kStreamC16; }
+static bool is_pc(int v) {
+  return v == kStreamC8 || v == kStreamC16 || v == kStreamC4 || v == kStreamC2;
+}
+
+// The streaming variant a precomputed-coefficient variant falls back to for
+// single steps.
+static int pc_base(int v) {
+  return v == kStreamC8 ? kStream8 : v == kStreamC4 ? kStream4 : v == kStreamC2 ? kStream2
+                                                                                 : kStream16;
+}
 
 static bool is_tb(int v) { return v >= kTb64x2 && v <= kTb32x4; }
 
@@ -311,16 +324,20 @@ static int tuned_variant(int H, int W) {
   return it == g_tuned.end() ? (int)kAuto : it->second;
 }
 
-// Default when no variant is set and the shape was not tuned: the streaming
-// kernel with 8-row bands, the fastest single-step kernel measured on MI355X
-// from 128^2 to 2048^2.
-static int resolve_variant(int v) { return v == kAuto ? (int)kStream8 : v; }
+// Default when no variant is set and the shape was not tuned: streaming bands
+// of 2 rows up to 1.5 M cells and of 4 rows above, the fastest single-step
+// kernels measured on MI355X from 256^2 to 2048^2 (small bands give enough
+// waves to hide load latency; large grids have enough waves already).
+static int default_variant(int H, int W) {
+  return (size_t)H * W <= 1536 * 1024 ? (int)kStream2 : (int)kStream4;
+}
+static int resolve_variant(int v) { return v == kAuto ? (int)kStream2 : v; }
 
 // The variant an automatic launch on an H x W grid uses.
 static int resolve_for(int v, int H, int W) {
   if (v != kAuto) return v;
   const int t = tuned_variant(H, W);
-  return t != kAuto ? t : (int)kStream8;
+  return t != kAuto ? t : default_variant(H, W);
 }
 
 template <int R>
@@ -384,6 +401,10 @@ static hipError_t launch_step(int variant, bool euler, const Coef& c, const floa
     case kStream32: launch_stream<32>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
     case kStreamC16:  // single steps: no coefficient precompute to amortise
     case kStream16: launch_stream<16>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
+    case kStreamC4:
+    case kStream4: launch_stream<4>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
+    case kStreamC2:
+    case kStream2: launch_stream<2>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
     case kStream8:
     default: launch_stream<8>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
   }
@@ -447,7 +468,7 @@ static int32_t forward_loop(int variant, const fk_params* params, const float* D
   float* wk = (float*)work;
   float* bufs[2][3] = {{v_out, w_out, u_out}, {wk, wk + n, wk + 2 * n}};
   int v = resolve_for(variant, H, W);
-  if (is_pc(v) && steps < 2) v = (v == kStreamC8) ? kStream8 : kStream16;  // nothing to amortise
+  if (is_pc(v) && steps < 2) v = pc_base(v);  // nothing to amortise
   float4* cf = nullptr;
   if (is_pc(v)) {
     cf = (float4*)((char*)work + coef_offset(H, W));
@@ -475,6 +496,10 @@ static int32_t forward_loop(int variant, const fk_params* params, const float* D
       const StimArgs& sa = sched.at(t + (double)k);
       if (v == kStreamC8)
         launch_stream_c<8>(c, cf, src[0], src[1], src[2], H, W, sa, dt, d[0], d[1], d[2], s);
+      else if (v == kStreamC4)
+        launch_stream_c<4>(c, cf, src[0], src[1], src[2], H, W, sa, dt, d[0], d[1], d[2], s);
+      else if (v == kStreamC2)
+        launch_stream_c<2>(c, cf, src[0], src[1], src[2], H, W, sa, dt, d[0], d[1], d[2], s);
       else
         launch_stream_c<16>(c, cf, src[0], src[1], src[2], H, W, sa, dt, d[0], d[1], d[2], s);
       HIP_CHECK(hipGetLastError());
@@ -586,10 +611,11 @@ int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  int best_v = kStream8;
+  int best_v = default_variant(H, W);
   float best_ms = 0.0f;
-  const int cand[] = {kStream8, kStream16, kStream32, kStreamC8, kStreamC16, kTile16,
-                      kTile32,  kTb64x2,   kTb64x4,   kTb32x2,   kTb32x4};
+  const int cand[] = {kStream8,  kStream4,  kStream2, kStream16, kStream32,
+                      kStreamC8, kStreamC4, kStreamC2, kStreamC16, kTile16,
+                      kTile32,   kTb64x2,   kTb64x4,  kTb32x2,   kTb32x4};
   if (times_ms)
     for (int k = 0; k < kNumVariants; ++k) times_ms[k] = -1.0f;
   for (int v : cand) {

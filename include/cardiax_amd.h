@@ -63,16 +63,22 @@ const char* fk_last_error(void);
 int32_t fk_abi_version(void);
 
 /* Kernel used for the forward step by fk_step, fk_step_euler and
- * fk_forward_euler: 0 = automatic (default: the autotuned choice for the
- * grid shape, else 4), 1 = plain one thread per cell, 2 / 3 = LDS tile of 16 /
- * 32 rows, 4 / 5 / 6 = streaming wavefront bands of 8 / 16 / 32 rows, 7 / 8 /
- * 9 / 10 = temporal blocking on 64x64 tiles (2 / 4 steps per launch) and 32x32
- * tiles (2 / 4 steps per launch), 11 / 12 = streaming bands of 8 / 16 rows
- * with the static diffusion coefficients precomputed once per
- * fk_forward_euler call. fk_step and fk_step_euler run variant 4 in place of
- * 7-11 and 5 in place of 12. Process-wide. No reference counterpart (the
- * reference leaves kernel choice to XLA). Returns the previous setting, or
- * FK_ERR_ARG for an unknown variant. */
+ * fk_forward_euler. No reference counterpart (the reference leaves kernel
+ * choice to XLA). Process-wide. Returns the previous setting, or FK_ERR_ARG
+ * for an unknown variant.
+ *    0      automatic: the autotuned choice for the grid shape, else 15 up to
+ *           1.5 M cells and 13 above
+ *    1      plain, one thread per cell
+ *    2, 3   LDS tile of 16 / 32 rows
+ *    4-6    streaming wavefront bands of 8 / 16 / 32 rows
+ *    7-10   temporal blocking: 64x64 tiles with 2 / 4 steps per launch, 32x32
+ *           tiles with 2 / 4 steps per launch
+ *    11, 12 streaming bands of 8 / 16 rows with the static diffusion
+ *           coefficients precomputed once per fk_forward_euler call
+ *    13, 15 streaming bands of 4 / 2 rows
+ *    14, 16 the same with precomputed coefficients
+ * fk_step and fk_step_euler run 4 in place of 7-11, 5 in place of 12, 13 in
+ * place of 14 and 15 in place of 16. */
 int32_t fk_set_forward_variant(int32_t variant);
 
 /* Autotuner of the forward step kernel. No reference counterpart (the
@@ -246,15 +252,16 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out);
 int32_t fk_batch_run(fk_batch* b, int64_t n_steps, int32_t use_graph, void* stream);
 
 /* Autotuner of the batch: times n_steps (1..10000) batched steps with bands
- * of 8 and of 16 rows on `stream` (synchronising it) and keeps the faster work
- * list for later runs. Overwrites only the batch's internal ping-pong pools,
- * never a simulation's state. times_ms (2 floats, may be NULL) and best_rows
- * (may be NULL) receive the timings and the chosen band height. */
+ * of 2, 4, 8 and 16 rows on `stream` (synchronising it) and keeps the fastest
+ * work list for later runs (8 rows until tuned). Overwrites only the batch's
+ * internal ping-pong pools, never a simulation's state. times_ms (4 floats,
+ * may be NULL) and best_rows (may be NULL) receive the timings and the chosen
+ * band height. */
 int32_t fk_batch_autotune(fk_batch* b, int32_t n_steps, void* stream, float* times_ms,
                           int32_t* best_rows);
 
-/* Pins the band height (8 or 16 rows) of later runs, as the autotuner would;
- * waits for the device first. */
+/* Pins the band height (2, 4, 8 or 16 rows) of later runs, as the autotuner
+ * would; waits for the device first. */
 int32_t fk_batch_set_rows(fk_batch* b, int32_t rows);
 
 /* Number of (simulation, strip, band) tiles of one batch step. */
