@@ -207,55 +207,71 @@ __device__ __forceinline__ void pre_load(const float* __restrict__ u,
 }
 
 // The derived fields of one cell, given its cotangent L (lam_u plus any
-// injected cotangent) and the loaded inputs.
+// injected cotangent) and the loaded inputs. lap is the axis-0 sum plus the
+// axis-1 sum, the order k_adj_main uses too.
+// Per-axis pieces of the derived fields of the previous step: A1 u and A2 u
+// at position i of an axis of N cells, from u at offsets -5..5 along the axis
+// (clamped into the grid; the coefficients there are 0).
+__device__ __forceinline__ void axis_pre(const float* __restrict__ xs, int i, int N,
+                                         const AdjAxisTab& T, float& g, float& lap) {
+  g = 0.0f;
+  lap = 0.0f;
+  if (central_row(i, N)) {
+#pragma unroll
+    for (int d = -4; d <= 4; ++d) {
+      lap += FK_A2(d) * xs[d + kAdjBand];
+      g += FK_A1(d) * xs[d + kAdjBand];
+    }
+  } else {
+#pragma unroll
+    for (int d = -kAdjBand; d <= kAdjBand; ++d) {
+      float a1, a2;
+      band_coef(T, i, d, N, a1, a2);
+      const bool on = i + d >= 0 && i + d < N;
+      g += on ? a1 * xs[d + kAdjBand] : 0.0f;
+      lap += on ? a2 * xs[d + kAdjBand] : 0.0f;
+    }
+  }
+}
+
+
 __device__ __forceinline__ void pre_store(const PreIn& in, float L, int i, int c, int H, int W,
-                                          const AdjAxisTab* __restrict__ tab,
-                                          const AdjPre& o) {
+                                          const AdjAxisTab* tab, const AdjPre& o) {
   const size_t idx = (size_t)i * W + c;
-  float gx = 0.0f, gy = 0.0f, lap = 0.0f;
-  if (central_row(i, H)) {
-#pragma unroll
-    for (int d = -4; d <= 4; ++d) {
-      lap += FK_A2(d) * in.xs[d + kAdjBand];
-      gx += FK_A1(d) * in.xs[d + kAdjBand];
-    }
-  } else {
-#pragma unroll
-    for (int d = -kAdjBand; d <= kAdjBand; ++d) {
-      float a1, a2;
-      band_coef(tab[0], i, d, H, a1, a2);
-      const bool on = i + d >= 0 && i + d < H;
-      gx += on ? a1 * in.xs[d + kAdjBand] : 0.0f;
-      lap += on ? a2 * in.xs[d + kAdjBand] : 0.0f;
-    }
-  }
-  if (central_row(c, W)) {
-#pragma unroll
-    for (int d = -4; d <= 4; ++d) {
-      lap += FK_A2(d) * in.ys[d + kAdjBand];
-      gy += FK_A1(d) * in.ys[d + kAdjBand];
-    }
-  } else {
-#pragma unroll
-    for (int d = -kAdjBand; d <= kAdjBand; ++d) {
-      float a1, a2;
-      band_coef(tab[1], c, d, W, a1, a2);
-      const bool on = c + d >= 0 && c + d < W;
-      gy += on ? a1 * in.ys[d + kAdjBand] : 0.0f;
-      lap += on ? a2 * in.ys[d + kAdjBand] : 0.0f;
-    }
-  }
-  o.Llap[idx] = make_float2(L, lap);
+  float gx, lapx, gy, lapy;
+  axis_pre(in.xs, i, H, tab[0], gx, lapx);
+  axis_pre(in.ys, c, W, tab[1], gy, lapy);
+  o.Llap[idx] = make_float2(L, lapx + lapy);
   o.QPx[idx] = make_float2(in.Dx * L, L * gx);
   o.QPy[idx] = make_float2(in.Dy * L, L * gy);
   o.Q2[idx] = in.D * L;
+}
+
+// The boundary tables of both axes staged in LDS by the whole block (one
+// barrier) when the block holds a cell within 13 cells of an edge: the edge
+// paths then read coefficients from LDS and only their field loads go to
+// memory, all independent of each other. Interior blocks (the condition is
+// block-uniform) skip the staging and never read the tables.
+__device__ __forceinline__ void stage_tables(const AdjAxisTab* __restrict__ tab, int H, int W,
+                                             int rows, AdjAxisTab* s_tab) {
+  const int r0 = blockIdx.y * rows, c0 = blockIdx.x * 64;
+  const bool interior = central_cone(r0, H) && central_cone(r0 + rows - 1, H) &&
+                        central_cone(c0, W) && central_cone(c0 + 63, W);
+  if (interior) return;
+  const float* src = (const float*)tab;
+  float* dst = (float*)s_tab;
+  constexpr int n = (int)(2 * sizeof(AdjAxisTab) / sizeof(float));
+  for (int k = threadIdx.x; k < n; k += blockDim.x) dst[k] = src[k];
+  __syncthreads();
 }
 
 __global__ void __launch_bounds__(kAdjThreads)
 k_adj_pre(const float* __restrict__ u, const float* __restrict__ D,
           const float* __restrict__ Dx, const float* __restrict__ Dy,
           const float* __restrict__ lu, const float* __restrict__ inj, int H, int W,
-          const AdjAxisTab* __restrict__ tab, AdjPre o) {
+          const AdjAxisTab* __restrict__ tab_g, AdjPre o) {
+  __shared__ AdjAxisTab tab[2];
+  stage_tables(tab_g, H, W, 4, tab);
   const int c = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= H || c >= W) return;
@@ -274,8 +290,48 @@ struct AdjNext {
   AdjPre pre;
 };
 
-// One reverse step from the derived fields. Each cell adds its contribution
-// to the tau gradients to its own entry of the three accumulator fields
+// Transposed operators along one axis at position i (stride between
+// neighbours `step` elements, base index `idx` of position i):
+// s = A2^T(Q2) + A1^T(Q1), e = A1^T(P), with (Q1, P) stored as pairs.
+__device__ __forceinline__ void axis_transpose(const float* __restrict__ Q2,
+                                               const float2* __restrict__ QP, size_t idx,
+                                               ptrdiff_t step, int i, int N, const AdjAxisTab& T,
+                                               float& sv, float& ev) {
+  sv = 0.0f;
+  ev = 0.0f;
+  if (central_cone(i, N)) {
+#pragma unroll
+    for (int d = -4; d <= 4; ++d) {
+      const size_t j = idx - (ptrdiff_t)d * step;
+      sv += FK_A2(d) * Q2[j];
+      if (d >= -2 && d <= 2 && d != 0) {
+        const float2 q = QP[j];
+        sv += FK_A1(d) * q.x;
+        ev += FK_A1(d) * q.y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int d = -kAdjBand; d <= kAdjBand; ++d) {
+      const int jr = i - d;
+      float a1, a2;
+      band_coef(T, jr, d, N, a1, a2);  // 0 when jr is off the axis
+      const size_t j = idx + (ptrdiff_t)(clampi(jr, 0, N - 1) - i) * step;
+      const float2 q = QP[j];
+      sv += a2 * Q2[j] + a1 * q.x;
+      ev += a1 * q.y;
+    }
+  }
+}
+
+// One reverse step from the derived fields, two threads per cell in
+// different waves of a block of 64 columns x 2 rows: the "x" thread loads and
+// applies the transposed operators along axis 0, evaluates the reaction's
+// adjoint and writes every output; the "y" thread applies those along axis 1
+// (and, with NEXT, derives the axis-1 part of the previous step's fields) and
+// hands its 4 results over through LDS (one barrier). Twice the waves and half
+// the dependency chain per wave: the step is latency-bound on small grids.
+// Each cell adds its tau-gradient contribution to its own accumulator
 // g_tau[cell].{x, y, z} (deterministic; summed by k_adj_reduce).
 // NEXT: also write the derived fields of the previous step's reverse (they
 // depend on the new cotangent only at the same cell), so a segment needs one
@@ -287,108 +343,107 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
            const float* __restrict__ lv, const float* __restrict__ lw, AdjPre pre,
            float* __restrict__ lv_o, float* __restrict__ lw_o, float* __restrict__ lu_o,
            float* __restrict__ gD, float4* __restrict__ g_tau,
-           const AdjAxisTab* __restrict__ tab, AdjNext nx) {
-  const int col = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i < H && col < W) {
-    const size_t idx = (size_t)i * W + col;
-    PreIn next{};
-    float inj = 0.0f;
+           const AdjAxisTab* __restrict__ tab_g, AdjNext nx) {
+  __shared__ AdjAxisTab tab[2];
+  __shared__ float4 xch[kAdjThreads / 2];  // y thread -> x thread of each cell
+  stage_tables(tab_g, H, W, 2, tab);
+  const bool ythread = threadIdx.x >= kAdjThreads / 2;  // wave-uniform
+  const int q = threadIdx.x & (kAdjThreads / 2 - 1);
+  const int col = blockIdx.x * 64 + (q & 63);
+  const int i = blockIdx.y * 2 + (q >> 6);
+  const bool in = i < H && col < W;
+  const size_t idx = in ? (size_t)i * W + col : 0;
+
+  // x-thread state (kept across the barrier)
+  float xs[2 * kAdjBand + 1];
+  float nD = 0.0f, nDx = 0.0f, nDy = 0.0f, inj = 0.0f;
+  float vv = 0.0f, ww = 0.0f, uu = 0.0f, lvv = 0.0f, lww = 0.0f, L = 0.0f, lap = 0.0f;
+  float sx = 0.0f, ex = 0.0f, gx_n = 0.0f, lapx_n = 0.0f;
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float gd = 0.0f;
+  bool stim = false;
+  if (ythread) {
+    float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (in) {
+      float ys[2 * kAdjBand + 1];
+      if (NEXT) {
+        const float* urow = nx.u_prev + (size_t)i * W;
+#pragma unroll
+        for (int d = -kAdjBand; d <= kAdjBand; ++d)
+          ys[d + kAdjBand] = urow[clampi(col + d, 0, W - 1)];
+      }
+      axis_transpose(pre.Q2, pre.QPy, idx, 1, col, W, tab[1], r.x, r.y);
+      if (NEXT) axis_pre(ys, col, W, tab[1], r.z, r.w);
+    }
+    xch[q] = r;
+  } else if (in) {
+    // x thread: everything that does not need the y results
     if (NEXT) {  // issued first: independent of everything below
-      pre_load(nx.u_prev, nx.D, nx.Dx, nx.Dy, i, col, H, W, next);
+#pragma unroll
+      for (int d = -kAdjBand; d <= kAdjBand; ++d)
+        xs[d + kAdjBand] = nx.u_prev[(size_t)clampi(i + d, 0, H - 1) * W + col];
+      nD = nx.D[idx];
+      nDx = nx.Dx[idx];
+      nDy = nx.Dy[idx];
       if (nx.inj) inj = nx.inj[idx];
     }
-    const float vv = v[idx], ww = w[idx], uu = u[idx];
+    vv = v[idx];
+    ww = w[idx];
+    uu = u[idx];
     const float2 Llap = pre.Llap[idx];
-    const float lvv = lv[idx], lww = lw[idx], L = Llap.x;
-
-    // --- reaction, gates as fixed masks ---------------------------------
-    const bool p = uu >= c.V_c;
-    const float inv_tvm = (uu >= c.V_v) ? c.inv_tv2m : c.inv_tv1m;
-    const float dfv_dv = p ? -c.inv_tvp : -inv_tvm;
-    const float dfw_dw = p ? -c.inv_twp : -c.inv_twm;
-    const float S = half_one_plus_tanh(c, uu);  // (1 + tanh(k (u - V_csi))) / 2
-    // dS/du = 2k S (1 - S); kk = -2 k log2(e)
-    const float dS = -c.kk * 0.69314718055994531f * S * (1.0f - S);
-    // j_ion = -(j_fi + j_so + j_si) / Cm
-    const float djdv = p ? (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_cm : 0.0f;
-    const float djdw = S * c.inv_tsi * c.inv_cm;
-    const float djdu =
-        -((p ? -vv * (1.0f + c.V_c - 2.0f * uu) * c.inv_td : 0.0f) + (p ? 0.0f : c.inv_t0) -
-          ww * dS * c.inv_tsi) *
-        c.inv_cm;
-    // stimulus replaces j_ion: no cotangent reaches it where active
-    bool stim = false;
+    L = Llap.x;
+    lap = Llap.y;
+    lvv = lv[idx];
+    lww = lw[idx];
+    gd = gD[idx];
+    acc = g_tau[idx];
     for (int k = 0; k < sa.n; ++k) stim = stim || (sa.f[k][idx] != 0.0f);
-    const float lus = stim ? 0.0f : L;
+    axis_transpose(pre.Q2, pre.QPx, idx, W, i, H, tab[0], sx, ex);
+    if (NEXT) axis_pre(xs, i, H, tab[0], gx_n, lapx_n);
+  }
+  __syncthreads();  // every thread of the block passes this one barrier
+  if (ythread || !in) return;
+  const float4 yr = xch[q];
+  const float sy = yr.x, ey = yr.y;
 
-    // --- transposed diffusion operators: (A^T q)(i) = sum_d A[i-d][i] q(i-d)
-    float sx = 0.0f, sy = 0.0f;  // A2^T(D L) + A1^T(D_? L)
-    float ex = 0.0f, ey = 0.0f;  // A1^T(L A1 u)
-    if (central_cone(i, H)) {
-#pragma unroll
-      for (int d = -4; d <= 4; ++d) {
-        const size_t j = (size_t)(i - d) * W + col;
-        sx += FK_A2(d) * pre.Q2[j];
-        if (d >= -2 && d <= 2 && d != 0) {
-          const float2 q = pre.QPx[j];
-          sx += FK_A1(d) * q.x;
-          ex += FK_A1(d) * q.y;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int d = -kAdjBand; d <= kAdjBand; ++d) {
-        const int jr = i - d;
-        float a1, a2;
-        band_coef(tab[0], jr, d, H, a1, a2);  // 0 when jr is off the axis
-        const size_t j = (size_t)clampi(jr, 0, H - 1) * W + col;
-        const float2 q = pre.QPx[j];
-        sx += a2 * pre.Q2[j] + a1 * q.x;
-        ex += a1 * q.y;
-      }
-    }
-    if (central_cone(col, W)) {
-#pragma unroll
-      for (int d = -4; d <= 4; ++d) {
-        const size_t j = idx - d;
-        sy += FK_A2(d) * pre.Q2[j];
-        if (d >= -2 && d <= 2 && d != 0) {
-          const float2 q = pre.QPy[j];
-          sy += FK_A1(d) * q.x;
-          ey += FK_A1(d) * q.y;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int d = -kAdjBand; d <= kAdjBand; ++d) {
-        const int jc = col - d;
-        float a1, a2;
-        band_coef(tab[1], jc, d, W, a1, a2);  // 0 when jc is off the axis
-        const size_t j = (size_t)i * W + clampi(jc, 0, W - 1);
-        const float2 q = pre.QPy[j];
-        sy += a2 * pre.Q2[j] + a1 * q.x;
-        ey += a1 * q.y;
-      }
-    }
+  // --- reaction, gates as fixed masks -----------------------------------
+  const bool p = uu >= c.V_c;
+  const float inv_tvm = (uu >= c.V_v) ? c.inv_tv2m : c.inv_tv1m;
+  const float dfv_dv = p ? -c.inv_tvp : -inv_tvm;
+  const float dfw_dw = p ? -c.inv_twp : -c.inv_twm;
+  const float S = half_one_plus_tanh(c, uu);  // (1 + tanh(k (u - V_csi))) / 2
+  // dS/du = 2k S (1 - S); kk = -2 k log2(e)
+  const float dS = -c.kk * 0.69314718055994531f * S * (1.0f - S);
+  // j_ion = -(j_fi + j_so + j_si) / Cm
+  const float djdv = p ? (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_cm : 0.0f;
+  const float djdw = S * c.inv_tsi * c.inv_cm;
+  const float djdu =
+      -((p ? -vv * (1.0f + c.V_c - 2.0f * uu) * c.inv_td : 0.0f) + (p ? 0.0f : c.inv_t0) -
+        ww * dS * c.inv_tsi) *
+      c.inv_cm;
+  // stimulus replaces j_ion: no cotangent reaches it where active
+  const float lus = stim ? 0.0f : L;
 
-    lv_o[idx] = lvv + dt * (dfv_dv * lvv + djdv * lus);
-    lw_o[idx] = lww + dt * (dfw_dw * lww + djdw * lus);
-    const float lu_new = L + dt * (djdu * lus + (sx + sy) * inv_dx2);
-    lu_o[idx] = lu_new;
-    if (NEXT) pre_store(next, lu_new + inj, i, col, H, W, tab, nx.pre);
-    gD[idx] += dt * (L * Llap.y + ex + ey) * inv_dx2;
-    // tau gradients: d j_ion / d tau
-    const float djd_td = p ? -vv * (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_td * c.inv_cm
-                           : 0.0f;
-    const float djd_tr = p ? c.inv_tr * c.inv_tr * c.inv_cm : 0.0f;
-    const float djd_tsi = -ww * S * c.inv_tsi * c.inv_tsi * c.inv_cm;
-    // per-cell accumulators of the tau gradients (reduced once at the end)
-    float4 acc = g_tau[idx];
-    acc.x += dt * djd_td * lus;
-    acc.y += dt * djd_tr * lus;
-    acc.z += dt * djd_tsi * lus;
-    g_tau[idx] = acc;
+  lv_o[idx] = lvv + dt * (dfv_dv * lvv + djdv * lus);
+  lw_o[idx] = lww + dt * (dfw_dw * lww + djdw * lus);
+  const float lu_new = L + dt * (djdu * lus + (sx + sy) * inv_dx2);
+  lu_o[idx] = lu_new;
+  gD[idx] = gd + dt * (L * lap + ex + ey) * inv_dx2;
+  // tau gradients: d j_ion / d tau
+  const float djd_td = p ? -vv * (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_td * c.inv_cm
+                         : 0.0f;
+  const float djd_tr = p ? c.inv_tr * c.inv_tr * c.inv_cm : 0.0f;
+  const float djd_tsi = -ww * S * c.inv_tsi * c.inv_tsi * c.inv_cm;
+  acc.x += dt * djd_td * lus;
+  acc.y += dt * djd_tr * lus;
+  acc.z += dt * djd_tsi * lus;
+  g_tau[idx] = acc;
+  if (NEXT) {  // derived fields of the previous step, at this cell
+    const float Ln = lu_new + inj;
+    nx.pre.Llap[idx] = make_float2(Ln, lapx_n + yr.w);
+    nx.pre.QPx[idx] = make_float2(nDx * Ln, Ln * gx_n);
+    nx.pre.QPy[idx] = make_float2(nDy * Ln, Ln * yr.z);
+    nx.pre.Q2[idx] = nD * Ln;
   }
 }
 
@@ -501,7 +556,7 @@ static hipError_t launch_main(const AdjLayout& L, char* wk, const Coef& c, const
                               int set, const float* u_prev, const float* inj, hipStream_t s) {
   const size_t n = (size_t)H * W;
   const AdjAxisTab* tab = (const AdjAxisTab*)(wk + L.tab);
-  const dim3 grid((W + 63) / 64, (H + 3) / 4);
+  const dim3 grid((W + 63) / 64, (H + 1) / 2);  // 64 x 2 cells, two threads each
   const AdjNext nx{u_prev, inj, D, (const float*)(wk + L.dx), (const float*)(wk + L.dy),
                    pre_fields(L, wk, 1 - set, n)};
   if (u_prev)
