@@ -16,7 +16,9 @@
  *   - Work is enqueued on `stream` (a hipStream_t; NULL = the null stream) and
  *     is asynchronous: no entry point synchronises the stream.
  *   - No entry point allocates device memory except fk_batch_create, which
- *     owns the device copy of the batch work list until fk_batch_destroy.
+ *     owns the device copy of the batch work list until fk_batch_destroy,
+ *     and fk_autotune_forward, which frees its timing scratch before it
+ *     returns.
  *   - Return value: FK_OK, or a negative FK_ERR_* code; fk_last_error()
  *     returns a thread-local message for the last failing call.
  */
