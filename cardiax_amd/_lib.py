@@ -27,7 +27,7 @@ FK_MAX_STIMULI = 32
 VARIANTS = {"auto": 0, "plain": 1, "tile16": 2, "tile32": 3, "stream8": 4, "stream16": 5,
             "stream32": 6, "tb64x2": 7, "tb64x4": 8, "tb32x2": 9, "tb32x4": 10,
             "streamc8": 11, "streamc16": 12, "stream4": 13, "streamc4": 14,
-            "stream2": 15, "streamc2": 16}
+            "stream2": 15, "streamc2": 16, "pair2": 17, "pair4": 18}
 
 c_f = ctypes.c_float
 c_d = ctypes.c_double
@@ -83,6 +83,13 @@ _SIGNATURES = {
     "fk_forward_trajectory": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32,
                                       c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d, c_i32,
                                       c_f, c_f, c_vp, c_vp]),
+    "fk_heun_workspace_bytes": (c_sz, [c_i32, c_i32]),
+    "fk_step_heun": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
+                             ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_f, c_vp, c_vp,
+                             c_vp, c_vp, c_sz, c_vp]),
+    "fk_forward_heun": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32,
+                                c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d, c_d, c_f, c_f,
+                                c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
     "fk_gradient": (c_i32, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp]),
     "fk_stimulate": (c_i32, [c_d, c_vp, c_vp, c_i64, ctypes.POINTER(FkStimulus), c_i32, c_vp]),
     "fk_adjoint_workspace_bytes": (c_sz, [c_i32, c_i32]),

@@ -25,6 +25,7 @@
 #include "fk_device.h"
 #include "fk_host.h"
 #include "fk_stream.h"
+#include "fk_stream2.h"
 #include "fk_tb.h"
 
 namespace fk {
@@ -294,7 +295,9 @@ enum Variant : int {
   kStreamC4 = 14,   // ... with precomputed coefficients
   kStream2 = 15,    // streaming bands of 2 rows
   kStreamC2 = 16,   // ... with precomputed coefficients
-  kNumVariants = 17
+  kPair2 = 17,      // streaming bands of 2 / 4 rows, two adjacent columns per lane
+  kPair4 = 18,      // (W odd: the one-column kernels 15 / 13)
+  kNumVariants = 19
 };
 
 static bool is_pc(int v) {
@@ -356,6 +359,20 @@ static void launch_stream(bool euler, const Coef& c, const float* D, const float
 }
 
 template <int R>
+static void launch_stream2(bool euler, const Coef& c, const float* D, const float* v,
+                           const float* w, const float* u, int H, int W, const StimArgs& sa,
+                           float dt, float inv_dx2, float* ov, float* ow, float* ou,
+                           hipStream_t s) {
+  const StreamGeom2 g = make_stream_geom2<R>(H, W);
+  if (euler)
+    k_stream2<R, true><<<g.blocks, 256, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow,
+                                               ou, g);
+  else
+    k_stream2<R, false><<<g.blocks, 256, 0, s>>>(c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow,
+                                                ou, g);
+}
+
+template <int R>
 static void launch_stream_c(const Coef& c, const float4* cf, const float* v, const float* w,
                             const float* u, int H, int W, const StimArgs& sa, float dt,
                             float* ov, float* ow, float* ou, hipStream_t s) {
@@ -405,6 +422,14 @@ static hipError_t launch_step(int variant, bool euler, const Coef& c, const floa
     case kStream4: launch_stream<4>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
     case kStreamC2:
     case kStream2: launch_stream<2>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
+    case kPair2:
+      if (W % 2 == 0) launch_stream2<2>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s);
+      else launch_stream<2>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s);
+      break;
+    case kPair4:
+      if (W % 2 == 0) launch_stream2<4>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s);
+      else launch_stream<4>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s);
+      break;
     case kStream8:
     default: launch_stream<8>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
   }
@@ -437,6 +462,57 @@ static hipError_t launch_tb_variant(int variant, const Coef& c, const float* D, 
     case kTb32x2: return launch_tb<32, 2, 256>(c, D, v, w, u, H, W, st, nsteps, dt, inv_dx2, ov, ow, ou, s);
     default: return launch_tb<32, 4, 512>(c, D, v, w, u, H, W, st, nsteps, dt, inv_dx2, ov, ow, ou, s);
   }
+}
+
+// One stage of a Heun step on the default band height for the shape.
+static hipError_t launch_heun(int stage, const Coef& c, const float* D, const float* const x[3],
+                              int H, int W, const StimArgs& sa, float dt, float inv_dx2,
+                              float* const o[3], const HeunIO& io, hipStream_t s) {
+  const bool two = default_variant(H, W) == kStream2;
+  const StreamGeom g = two ? make_stream_geom<2>(H, W) : make_stream_geom<4>(H, W);
+#define FK_HEUN(R, ST)                                                                     \
+  k_stream_heun<R, ST><<<g.blocks, 256, 0, s>>>(c, D, x[0], x[1], x[2], H, W, sa, dt, inv_dx2, \
+                                                o[0], o[1], o[2], g, io)
+  if (two) {
+    if (stage == 1) FK_HEUN(2, 1); else FK_HEUN(2, 2);
+  } else {
+    if (stage == 1) FK_HEUN(4, 1); else FK_HEUN(4, 2);
+  }
+#undef FK_HEUN
+  return hipGetLastError();
+}
+
+// Heun steps at times t .. t_end-1 (cardiax/solve.py:103-111) from `in` to
+// `out`; the workspace holds x1 and k1 (6 fields) and a ping-pong state.
+static int32_t heun_loop(const fk_params* params, const float* D, const float* const in[3],
+                         int32_t H, int32_t W, const fk_stimulus* stimuli, int32_t n_stimuli,
+                         double t, int64_t steps, float dt, float dx, float* const out[3],
+                         void* work, hipStream_t s) {
+  const size_t n = (size_t)H * W;
+  const Coef c = make_coef(*params);
+  const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
+  float* wk = (float*)work;
+  float* x1[3] = {wk, wk + n, wk + 2 * n};
+  float* k1[3] = {wk + 3 * n, wk + 4 * n, wk + 5 * n};
+  float* ping[3] = {wk + 6 * n, wk + 7 * n, wk + 8 * n};
+  float* const* bufs[2] = {out, ping};
+  int dst = (steps % 2 == 1) ? 0 : 1;  // the last step lands in `out`
+  const float* src[3] = {in[0], in[1], in[2]};
+  StimSchedule sched(stimuli, n_stimuli);
+  for (int64_t k = 0; k < steps; ++k) {
+    const StimArgs& sa = sched.at(t + (double)k);  // both stages at time t + k
+    HeunIO io1{k1[0], k1[1], k1[2], nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    HIP_CHECK(launch_heun(1, c, D, src, H, W, sa, dt, inv_dx2, x1, io1, s));
+    HeunIO io2{nullptr, nullptr, nullptr, src[0], src[1], src[2], k1[0], k1[1], k1[2]};
+    const float* x1c[3] = {x1[0], x1[1], x1[2]};
+    float* const* d = bufs[dst];
+    HIP_CHECK(launch_heun(2, c, D, x1c, H, W, sa, dt, inv_dx2, d, io2, s));
+    src[0] = d[0];
+    src[1] = d[1];
+    src[2] = d[2];
+    dst ^= 1;
+  }
+  return FK_OK;
 }
 
 static int32_t forward_loop(int variant, const fk_params* params, const float* D,
@@ -613,9 +689,9 @@ int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
   if (e == hipSuccess) e = hipEventCreate(&e1);
   int best_v = default_variant(H, W);
   float best_ms = 0.0f;
-  const int cand[] = {kStream8,  kStream4,  kStream2, kStream16, kStream32,
-                      kStreamC8, kStreamC4, kStreamC2, kStreamC16, kTile16,
-                      kTile32,   kTb64x2,   kTb64x4,  kTb32x2,   kTb32x4};
+  const int cand[] = {kStream8,  kStream4,  kStream2,  kStream16, kStream32, kStreamC8,
+                      kStreamC4, kStreamC2, kStreamC16, kPair2,   kPair4,    kTile16,
+                      kTile32,   kTb64x2,   kTb64x4,   kTb32x2,  kTb32x4};
   if (times_ms)
     for (int k = 0; k < kNumVariants; ++k) times_ms[k] = -1.0f;
   for (int v : cand) {
@@ -698,6 +774,57 @@ int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const fl
                                void* stream) {
   return forward_loop(kPlain, params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli,
                       t, t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
+}
+
+size_t fk_heun_workspace_bytes(int32_t H, int32_t W) {
+  if (H <= 0 || W <= 0) return 0;
+  return 9 * (size_t)H * (size_t)W * sizeof(float);
+}
+
+int32_t fk_step_heun(const fk_params* params, const float* D, const float* v, const float* w,
+                     const float* u, int32_t H, int32_t W, const fk_stimulus* stimuli,
+                     int32_t n_stimuli, double t, float dt, float dx, float* v_out,
+                     float* w_out, float* u_out, void* work, size_t work_bytes, void* stream) {
+  ErrorScope es;
+  if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
+      !check_ptrs({D, v, w, u, v_out, w_out, u_out, work}) || !check_scalar("t", t) ||
+      !check_scalar("dt", dt) || !check_dx(dx))
+    return last_code();
+  if (work_bytes < fk_heun_workspace_bytes(H, W))
+    return set_error(FK_ERR_ARG, "workspace too small: need %zu bytes, got %zu",
+                     fk_heun_workspace_bytes(H, W), work_bytes);
+  const float* in[3] = {v, w, u};
+  float* out[3] = {v_out, w_out, u_out};
+  return heun_loop(params, D, in, H, W, stimuli, n_stimuli, t, 1, dt, dx, out, work,
+                   (hipStream_t)stream);
+}
+
+int32_t fk_forward_heun(const fk_params* params, const float* D, const float* v_in,
+                        const float* w_in, const float* u_in, int32_t H, int32_t W,
+                        const fk_stimulus* stimuli, int32_t n_stimuli, double t, double t_end,
+                        float dt, float dx, float* v_out, float* w_out, float* u_out,
+                        void* work, size_t work_bytes, void* stream) {
+  ErrorScope es;
+  if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
+      !check_ptrs({D, v_in, w_in, u_in, v_out, w_out, u_out}) || !check_time(t, t_end) ||
+      !check_scalar("dt", dt) || !check_dx(dx))
+    return last_code();
+  const int64_t steps = loop_count(t, t_end);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)H * W;
+  if (steps == 0) {
+    for (auto pr : {std::make_pair(v_in, v_out), std::make_pair(w_in, w_out),
+                    std::make_pair(u_in, u_out)})
+      HIP_CHECK(hipMemcpyAsync(pr.second, pr.first, n * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
+    return FK_OK;
+  }
+  if (work == nullptr || work_bytes < fk_heun_workspace_bytes(H, W))
+    return set_error(FK_ERR_ARG, "workspace too small: need %zu bytes, got %zu",
+                     fk_heun_workspace_bytes(H, W), work_bytes);
+  const float* in[3] = {v_in, w_in, u_in};
+  float* out[3] = {v_out, w_out, u_out};
+  return heun_loop(params, D, in, H, W, stimuli, n_stimuli, t, steps, dt, dx, out, work, s);
 }
 
 int32_t fk_gradient(const float* a, float* out, int32_t rows, int32_t cols, int32_t axis,

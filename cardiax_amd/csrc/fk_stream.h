@@ -116,9 +116,21 @@ struct StimListView {
   __device__ const float* field(int m) const { return sa.f[m]; }
 };
 
+// The extra inputs and outputs of the two stages of a Heun step
+// (cardiax/solve.py:73-85). Stage 1 also writes the slope k1 = step(x) to k;
+// stage 2, evaluated on x1 = x + dt k1, reads x0 = x and k1 and writes
+// x0 + (k1 + k2) dt / 2.
+struct HeunIO {
+  float *kv, *kw, *ku;              // stage 1: k1 out
+  const float *v0, *w0, *u0;        // stage 2: x0 in
+  const float *k1v, *k1w, *k1u;     // stage 2: k1 in
+};
+
 // PC: the static diffusion coefficients come precomputed from `cf`
-// (k_stream_coef) instead of being derived from D each step.
-template <int R, bool EULER, bool XG, bool YG, bool STIM, class SV, bool PC = false>
+// (k_stream_coef) instead of being derived from D each step. HEUN: 0 = an
+// Euler step (EULER) or the derivatives, 1 / 2 = stage 1 / 2 of a Heun step.
+template <int R, bool EULER, bool XG, bool YG, bool STIM, class SV, bool PC = false,
+          int HEUN = 0>
 __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restrict__ D,
                                             const float4* __restrict__ cf,
                                             const float* __restrict__ v,
@@ -126,7 +138,8 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
                                             const float* __restrict__ u, int H, int W,
                                             const SV& sv, float dt, float inv_dx2,
                                             float* __restrict__ o_v, float* __restrict__ o_w,
-                                            float* __restrict__ o_u, int rs, int cs) {
+                                            float* __restrict__ o_u, int rs, int cs,
+                                            const HeunIO& io = HeunIO{}) {
 #pragma clang fp contract(off)  // explicit fmaf only: identical rounding in every kernel
   const int lane = threadIdx.x & 63;
   const int col = cs - 4 + lane;                           // this lane's column
@@ -154,6 +167,18 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
   for (int k = 0; k < R; ++k) {
     vv[k] = ld(row(v, rs + k), off);
     ww[k] = ld(row(w, rs + k), off);
+  }
+  float h0[HEUN == 2 ? 3 * R : 1], hk[HEUN == 2 ? 3 * R : 1];  // stage 2: x0, k1
+  if constexpr (HEUN == 2) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      h0[3 * k] = ld(row(io.v0, rs + k), off);
+      h0[3 * k + 1] = ld(row(io.w0, rs + k), off);
+      h0[3 * k + 2] = ld(row(io.u0, rs + k), off);
+      hk[3 * k] = ld(row(io.k1v, rs + k), off);
+      hk[3 * k + 1] = ld(row(io.k1w, rs + k), off);
+      hk[3 * k + 2] = ld(row(io.k1u, rs + k), off);
+    }
   }
 
   // 12 u_x at rows q = rs-2 .. rs+R+1 (every difference below is scaled by
@@ -213,7 +238,19 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
     const float du = del_u + jion;
     if (out_col && (!XG || i < H)) {  // rows past the grid only when H < R
       const size_t ro = (size_t)i * (size_t)W;
-      if (EULER) {
+      if constexpr (HEUN == 1) {  // x1 = x + dt k1, and k1
+        st(o_v + ro, off, fmaf(dt, dv, vv[k]));
+        st(o_w + ro, off, fmaf(dt, dw, ww[k]));
+        st(o_u + ro, off, fmaf(dt, du, uc));
+        st(io.kv + ro, off, dv);
+        st(io.kw + ro, off, dw);
+        st(io.ku + ro, off, du);
+      } else if constexpr (HEUN == 2) {  // x0 + (k1 + k2) dt / 2
+        const float half = dt * 0.5f;
+        st(o_v + ro, off, fmaf(hk[3 * k] + dv, half, h0[3 * k]));
+        st(o_w + ro, off, fmaf(hk[3 * k + 1] + dw, half, h0[3 * k + 1]));
+        st(o_u + ro, off, fmaf(hk[3 * k + 2] + du, half, h0[3 * k + 2]));
+      } else if (EULER) {
         st(o_v + ro, off, fmaf(dt, dv, vv[k]));
         st(o_w + ro, off, fmaf(dt, dw, ww[k]));
         st(o_u + ro, off, fmaf(dt, du, uc));
@@ -229,12 +266,13 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
 // One wave per (strip, band) tile, band-major within a strip so that
 // consecutive tiles share row halos; four waves per block. PC: Euler steps
 // from precomputed coefficients (k_stream_c), else derived from D.
-template <int R, bool EULER, bool PC>
+template <int R, bool EULER, bool PC, int HEUN = 0>
 __device__ __forceinline__ void stream_kernel_body(
     const Coef& c, const float* __restrict__ D, const float4* __restrict__ cf,
     const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ u,
     int H, int W, const StimArgs& sa, float dt, float inv_dx2, float* __restrict__ o_v,
-    float* __restrict__ o_w, float* __restrict__ o_u, const StreamGeom& g) {
+    float* __restrict__ o_w, float* __restrict__ o_u, const StreamGeom& g,
+    const HeunIO& io = HeunIO{}) {
   const int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x >> 6));
   if (tile >= g.n_tiles) return;  // whole wave leaves; the kernel has no barrier
   const int strip = tile / g.n_bands, band = tile - strip * g.n_bands;
@@ -244,9 +282,9 @@ __device__ __forceinline__ void stream_kernel_body(
   const bool yg = cs < 3 || cs + kStripOut + 3 > W;  // strip within 3 columns
   const int sel = (xg ? 1 : 0) | (yg ? 2 : 0) | (sa.n ? 4 : 0);
   const StimListView sv{sa};
-#define FK_TILE(X, Y, S)                                                                  \
-  stream_tile<R, EULER, X, Y, S, StimListView, PC>(c, D, cf, v, w, u, H, W, sv, dt, inv_dx2, \
-                                                   o_v, o_w, o_u, rs, cs)
+#define FK_TILE(X, Y, S)                                                                 \
+  stream_tile<R, EULER, X, Y, S, StimListView, PC, HEUN>(c, D, cf, v, w, u, H, W, sv, dt,  \
+                                                         inv_dx2, o_v, o_w, o_u, rs, cs, io)
   switch (sel) {
     case 0: FK_TILE(false, false, false); break;
     case 1: FK_TILE(true, false, false); break;
@@ -268,6 +306,17 @@ k_stream(Coef c, const float* __restrict__ D, const float* __restrict__ v,
          float* __restrict__ o_u, StreamGeom g) {
   stream_kernel_body<R, EULER, false>(c, D, nullptr, v, w, u, H, W, sa, dt, inv_dx2, o_v, o_w,
                                       o_u, g);
+}
+
+// Stage STAGE (1 or 2) of a Heun step on streaming bands of R rows.
+template <int R, int STAGE>
+__global__ void __launch_bounds__(256)
+k_stream_heun(Coef c, const float* __restrict__ D, const float* __restrict__ v,
+              const float* __restrict__ w, const float* __restrict__ u, int H, int W,
+              StimArgs sa, float dt, float inv_dx2, float* __restrict__ o_v,
+              float* __restrict__ o_w, float* __restrict__ o_u, StreamGeom g, HeunIO io) {
+  stream_kernel_body<R, true, false, STAGE>(c, D, nullptr, v, w, u, H, W, sa, dt, inv_dx2, o_v,
+                                            o_w, o_u, g, io);
 }
 
 template <int R>

@@ -139,15 +139,36 @@ def step_euler(state, t, params, diffusivity, stimuli, dt, dx) -> State:
     return out
 
 
-def _forward_impl(entry, state, t, t_end, params, diffusivity, stimuli, dt, dx) -> State:
-    dev = require_gpu("_forward_euler")
+def step_heun(state, t, params, diffusivity, stimuli, dt, dx) -> State:
+    """cardiax/solve.py:73-85: k1 = step(x, t), k2 = step(x + dt k1, t),
+    x + (k1 + k2) dt / 2, as two fused launches (fk_step_heun)."""
+    dev = require_gpu("step_heun")
     st = _dev_state(state, dev)
     H, W = _shape_of(st)
     D = _dev_field(diffusivity, dev, (H, W), "diffusivity")
     stim = _Stimuli(stimuli, dev, (H, W))
     out = State(*(torch.empty_like(st.u) for _ in range(3)))
     L = _lib.lib()
-    nbytes = L.fk_forward_workspace_bytes(H, W)
+    nbytes = L.fk_heun_workspace_bytes(H, W)
+    work = torch.empty(max(1, nbytes // 4), dtype=torch.float32, device=dev)
+    p = _lib.make_params(params)
+    _lib.check(L.fk_step_heun(
+        ctypes.byref(p), D.data_ptr(), st.v.data_ptr(), st.w.data_ptr(), st.u.data_ptr(), H,
+        W, stim.pointer(), stim.n, _scalar(t), float(dt), float(dx), out.v.data_ptr(),
+        out.w.data_ptr(), out.u.data_ptr(), work.data_ptr(), nbytes, stream_handle()))
+    return out
+
+
+def _forward_impl(entry, state, t, t_end, params, diffusivity, stimuli, dt, dx,
+                  workspace="fk_forward_workspace_bytes") -> State:
+    dev = require_gpu(entry)
+    st = _dev_state(state, dev)
+    H, W = _shape_of(st)
+    D = _dev_field(diffusivity, dev, (H, W), "diffusivity")
+    stim = _Stimuli(stimuli, dev, (H, W))
+    out = State(*(torch.empty_like(st.u) for _ in range(3)))
+    L = _lib.lib()
+    nbytes = getattr(L, workspace)(H, W)
     work = torch.empty(max(1, nbytes // 4), dtype=torch.float32, device=dev)
     p = _lib.make_params(params)
     _lib.check(getattr(L, entry)(
@@ -180,7 +201,13 @@ def _unsupported_integrator(name):
     return f
 
 
-_forward_heun = _unsupported_integrator("HEUN")
+def _forward_heun(state, t, t_end, params, diffusivity, stimuli, dt, dx) -> State:
+    """cardiax/solve.py:103-111: Heun steps for i = t, t+1, ... while i < t_end;
+    both stages of step i use stimulus time i."""
+    return _forward_impl("fk_forward_heun", state, t, t_end, params, diffusivity, stimuli,
+                         dt, dx, workspace="fk_heun_workspace_bytes")
+
+
 _forward_dormandprince = _unsupported_integrator("DORMANDPRINCE")
 
 

@@ -79,6 +79,8 @@ int32_t fk_abi_version(void);
  *           coefficients precomputed once per fk_forward_euler call
  *    13, 15 streaming bands of 4 / 2 rows
  *    14, 16 the same with precomputed coefficients
+ *    17, 18 streaming bands of 2 / 4 rows with two adjacent columns per lane
+ *           (for odd W: 15 / 13)
  * fk_step and fk_step_euler run 4 in place of 7-11, 5 in place of 12, 13 in
  * place of 14 and 15 in place of 16. */
 int32_t fk_set_forward_variant(int32_t variant);
@@ -142,6 +144,28 @@ int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const fl
                                double t_end, float dt, float dx, float* v_out,
                                float* w_out, float* u_out, void* work, size_t work_bytes,
                                void* stream);
+
+/* Scratch bytes fk_step_heun and fk_forward_heun need for an H x W grid
+ * (the stage-1 state and slope, and a ping-pong state: 9 fields). */
+size_t fk_heun_workspace_bytes(int32_t H, int32_t W);
+
+/* cardiax/solve.py:73-85 -- one Heun step at time t: k1 = step(x, t),
+ * x1 = x + dt k1, k2 = step(x1, t), out = x + (k1 + k2) dt / 2. Two fused
+ * launches (the first writes x1 and k1 to the workspace). Outputs must not
+ * alias inputs or the workspace. */
+int32_t fk_step_heun(const fk_params* params, const float* D, const float* v, const float* w,
+                     const float* u, int32_t H, int32_t W, const fk_stimulus* stimuli,
+                     int32_t n_stimuli, double t, float dt, float dx, float* v_out,
+                     float* w_out, float* u_out, void* work, size_t work_bytes, void* stream);
+
+/* cardiax/solve.py:103-111 -- Heun steps for i = t, t+1, ... while i < t_end,
+ * step i at stimulus time i; contract as fk_forward_euler with the workspace
+ * of fk_heun_workspace_bytes. */
+int32_t fk_forward_heun(const fk_params* params, const float* D, const float* v_in,
+                        const float* w_in, const float* u_in, int32_t H, int32_t W,
+                        const fk_stimulus* stimuli, int32_t n_stimuli, double t, double t_end,
+                        float dt, float dx, float* v_out, float* w_out, float* u_out,
+                        void* work, size_t work_bytes, void* stream);
 
 /* The states of cardiax/solve.py:92-100 after each of n_steps Euler steps
  * from (v0, w0, u0) at time t: state k (k = 0 .. n_steps-1) is the state after

@@ -13,6 +13,8 @@ Reference behaviour followed (file:line under the reference tree):
   * ``State``/``init``        cardiax/solve.py:12-23
   * ``step``                  cardiax/solve.py:26-65
   * ``step_euler``            cardiax/solve.py:68-70
+  * ``step_heun``             cardiax/solve.py:73-85
+  * ``forward_heun``          cardiax/solve.py:103-111
   * ``forward_euler``         cardiax/solve.py:92-100 (fori_loop over [t, t_end))
   * ``forward``               cardiax/solve.py:168-222 (host segment loop)
   * ``gradient``              cardiax/solve.py:225-254
@@ -162,6 +164,25 @@ def step_euler(state: State, t, params, diffusivity, stimuli, dt, dx) -> State:
     g = step(state, t, params, diffusivity, stimuli, dx)
     dtt = state.u.dtype.type(dt)
     return State(*(x + dxdt * dtt for x, dxdt in zip(state, g)))
+
+
+def step_heun(state: State, t, params, diffusivity, stimuli, dt, dx) -> State:
+    """cardiax/solve.py:73-85: k1 = step(x, t), x1 = x + dt k1, k2 = step(x1, t)
+    (the same stimulus time), x + (k1 + k2) * (dt / 2)."""
+    dtt = state.u.dtype.type(dt)
+    k1 = step(state, t, params, diffusivity, stimuli, dx)
+    x1 = State(*(x + d * dtt for x, d in zip(state, k1)))
+    k2 = step(x1, t, params, diffusivity, stimuli, dx)
+    half = dtt * state.u.dtype.type(0.5)
+    return State(*(x + (a + b) * half for x, a, b in zip(state, k1, k2)))
+
+
+def forward_heun(state: State, t, t_end, params, diffusivity, stimuli, dt, dx) -> State:
+    """cardiax/solve.py:103-111: Heun steps for i in [t, t_end), step i at
+    stimulus time i."""
+    for i in loop_times(t, t_end):
+        state = step_heun(state, i, params, diffusivity, stimuli, dt, dx)
+    return state
 
 
 def loop_times(t: float, t_end: float):

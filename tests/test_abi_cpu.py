@@ -107,12 +107,19 @@ def test_no_cpu_fallback():
 
 def test_time_integrator_members_are_the_callables():
     assert solve.TimeIntegrator.EULER is solve._forward_euler
+    assert solve.TimeIntegrator.HEUN is solve._forward_heun
+    st = (np.ones((8, 8)), np.ones((8, 8)), np.zeros((8, 8)))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        solve.TimeIntegrator.HEUN(st, 0, 10, PARAMSET_3, np.full((8, 8), 1e-3), [], 0.01, 0.01)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        solve.step_heun(st, 0, PARAMSET_3, np.full((8, 8), 1e-3), [], 0.01, 0.01)
     with pytest.raises(NotImplementedError):
         solve.TimeIntegrator.DORMANDPRINCE(None, [0, 1], None, None, None, 0.01, 0.01)
 
 
 def test_public_surface_matches_reference_names():
-    for name in ("State", "init", "step", "step_euler", "_forward_euler", "TimeIntegrator",
+    for name in ("State", "init", "step", "step_euler", "step_heun", "_forward_euler",
+                 "_forward_heun", "TimeIntegrator",
                  "forward", "forward_dimensional", "gradient", "stimulate"):
         assert hasattr(solve, name), name
     assert solve.State._fields == ("v", "w", "u")

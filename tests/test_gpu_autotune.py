@@ -39,8 +39,8 @@ def test_forward_autotune_records_and_is_used(gpu):
         res = autotune.tune(shape, n_steps=20)
         timed = set(res["times_ms"])
         assert timed == {"stream2", "stream4", "stream8", "stream16", "stream32", "streamc2",
-                         "streamc4", "streamc8", "streamc16", "tile16", "tile32", "tb64x2",
-                         "tb64x4", "tb32x2", "tb32x4"}
+                         "streamc4", "streamc8", "streamc16", "pair2", "pair4", "tile16",
+                         "tile32", "tb64x2", "tb64x4", "tb32x2", "tb32x4"}
         assert all(t > 0 for t in res["times_ms"].values())
         assert res["best"] == min(res["times_ms"], key=res["times_ms"].get)
         assert autotune.lookup(shape) == res["best"]
