@@ -32,7 +32,8 @@ KERNELS = {"stream8": ("k_stream<8, true>", 1), "stream16": ("k_stream<16, true>
            "tb32x4": ("k_tb<32, 4, 512>", 4), "streamc8": ("k_stream_c<8>", 1),
            "streamc16": ("k_stream_c<16>", 1), "stream4": ("k_stream<4, true>", 1),
            "streamc4": ("k_stream_c<4>", 1), "stream2": ("k_stream<2, true>", 1),
-           "streamc2": ("k_stream_c<2>", 1)}
+           "streamc2": ("k_stream_c<2>", 1), "pair2": ("k_stream2<2, true>", 1),
+           "pair4": ("k_stream2<4, true>", 1)}
 
 
 def log(*a):
@@ -285,8 +286,7 @@ def main():
     if not args.no_tune:  # setup, outside the timed region
         tuned = autotune.tune((H, W), n_steps=300)
         log("autotune %dx%d: %s" % (H, W, json.dumps(tuned)))
-    variant = autotune.lookup((H, W))
-    variant = "stream8" if variant == "auto" else variant
+    variant = autotune.resolved((H, W))
 
     def run():
         return solve._forward_euler(st0, sim.t0, sim.t1, sim.params, D, stims, sim.dt, sim.dx)

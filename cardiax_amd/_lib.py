@@ -66,6 +66,7 @@ _SIGNATURES = {
     "fk_autotune_forward": (c_i32, [c_i32, c_i32, c_i32, c_vp, ctypes.POINTER(c_f),
                                     ctypes.POINTER(c_i32)]),
     "fk_autotune_lookup": (c_i32, [c_i32, c_i32]),
+    "fk_forward_variant_for": (c_i32, [c_i32, c_i32]),
     "fk_autotune_reset": (None, []),
     "fk_init": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "fk_step": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,

@@ -38,6 +38,16 @@ def lookup(shape) -> str:
     return NAMES[int(_lib.lib().fk_autotune_lookup(H, W))]
 
 
+def resolved(shape) -> str:
+    """Name of the variant that forward solves of ``shape`` run now: the
+    process-wide setting, else the tuned one, else the size-based default."""
+    H, W = (int(x) for x in shape)
+    v = int(_lib.lib().fk_forward_variant_for(H, W))
+    if v < 0:
+        _lib.check(v)
+    return NAMES[v]
+
+
 def reset() -> None:
     """Forget every tuned shape."""
     _lib.lib().fk_autotune_reset()

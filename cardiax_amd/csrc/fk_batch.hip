@@ -238,7 +238,7 @@ int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out) {
           t.sim = i;
           t.cs = std::min(kStripOut * strip, g.cs_last);
           t.rs = std::min(R * band, g.rs_last);
-          const bool xg = t.rs < 3 || t.rs + R + 3 > s.H;
+          const bool xg = edge_band(t.rs, R, s.H, s.W);
           const bool yg = t.cs < 3 || t.cs + kStripOut + 3 > s.W;
           t.flags = (xg ? 1 : 0) | (yg ? 2 : 0);
           items[o].push_back(t);

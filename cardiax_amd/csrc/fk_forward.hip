@@ -327,12 +327,15 @@ static int tuned_variant(int H, int W) {
   return it == g_tuned.end() ? (int)kAuto : it->second;
 }
 
-// Default when no variant is set and the shape was not tuned: streaming bands
-// of 2 rows up to 1.5 M cells and of 4 rows above, the fastest single-step
-// kernels measured on MI355X from 256^2 to 2048^2 (small bands give enough
-// waves to hide load latency; large grids have enough waves already).
+// Default when no variant is set and the shape was not tuned, from the
+// variants measured on MI355X: streaming bands of 2 rows up to 1.5 M cells
+// and of 4 rows up to 8 M (small bands give enough waves to hide load latency
+// while the grid sits in the 256 MB MALL), and the LDS tile of 16 rows above
+// (grids in DRAM, where the bands' halo re-reads cost DRAM bandwidth and a
+// tile stages its halo once).
 static int default_variant(int H, int W) {
-  return (size_t)H * W <= 1536 * 1024 ? (int)kStream2 : (int)kStream4;
+  const size_t n = (size_t)H * W;
+  return n <= 1536 * 1024 ? (int)kStream2 : n <= 8 * 1024 * 1024 ? (int)kStream4 : (int)kTile16;
 }
 static int resolve_variant(int v) { return v == kAuto ? (int)kStream2 : v; }
 
@@ -734,6 +737,15 @@ int32_t fk_autotune_lookup(int32_t H, int32_t W) { return tuned_variant(H, W); }
 void fk_autotune_reset(void) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned.clear();
+}
+
+int32_t fk_forward_variant_for(int32_t H, int32_t W) {
+  ErrorScope es;
+  if (H < 3 || W < 3) return set_error(FK_ERR_ARG, "grid %d x %d: H, W must be >= 3", H, W);
+  const int v = resolve_for(g_variant.load(), H, W);
+  if (W % 2 && v == kPair2) return kStream2;
+  if (W % 2 && v == kPair4) return kStream4;
+  return v;
 }
 
 int32_t fk_forward_trajectory(const fk_params* params, const float* D, const float* v0,

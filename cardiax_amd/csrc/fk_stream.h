@@ -145,39 +145,51 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
   const int col = cs - 4 + lane;                           // this lane's column
   const uint32_t off = (uint32_t)clampi(col, 0, W - 1) * 4u;  // clamped = edge padding
   const bool out_col = lane >= 4 && lane < 4 + kStripOut && col < W;
-  auto row = [&](const float* a, int r) {  // wave-uniform row base, clamped
-    return a + (size_t)clampi(r, 0, H - 1) * (size_t)W;
+  // Row rs - 4 + k of a field. Edge bands clamp the row (edge padding) and
+  // address it from a scalar row base. Interior bands (edge_band false) add
+  // one scalar band offset to each field base and a 32-bit per-row offset,
+  // k * 4W, that every field shares: a vector add per row instead of a scalar
+  // clamp, 64-bit multiply and add per row and field.
+  const uint32_t rb = (uint32_t)W * 4u;
+  const size_t band = XG ? 0 : (size_t)(rs - 4) * (size_t)W * 4u;
+  auto ldk = [&](const float* a, int k) {
+    if constexpr (XG) return ld(a + (size_t)clampi(rs - 4 + k, 0, H - 1) * (size_t)W, off);
+    else return ld((const float*)((const char*)a + band), off + (uint32_t)k * rb);
+  };
+  auto stk = [&](float* a, int k, float x) {  // rows inside the grid only
+    if constexpr (XG) st(a + (size_t)(rs - 4 + k) * (size_t)W, off, x);
+    else st((float*)((char*)a + band), off + (uint32_t)k * rb, x);
   };
 
   float uu[R + 8];           // u rows rs-4 .. rs+R+3
   float dd[PC ? 1 : R + 4];  // D rows rs-2 .. rs+R+1
   float4 cc[PC ? R : 1];     // precomputed coefficients of rows rs .. rs+R-1
 #pragma unroll
-  for (int k = 0; k < R + 8; ++k) uu[k] = ld(row(u, rs - 4 + k), off);
+  for (int k = 0; k < R + 8; ++k) uu[k] = ldk(u, k);
   if constexpr (PC) {
 #pragma unroll
     for (int k = 0; k < R; ++k)
       cc[k] = ld4(cf + (size_t)clampi(rs + k, 0, H - 1) * (size_t)W, off * 4u);
   } else {
 #pragma unroll
-    for (int k = 0; k < R + 4; ++k) dd[k] = ld(row(D, rs - 2 + k), off);
+    for (int k = 0; k < R + 4; ++k) dd[k] = ldk(D, k + 2);
   }
   float vv[R], ww[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    vv[k] = ld(row(v, rs + k), off);
-    ww[k] = ld(row(w, rs + k), off);
+    vv[k] = ldk(v, k + 4);
+    ww[k] = ldk(w, k + 4);
   }
   float h0[HEUN == 2 ? 3 * R : 1], hk[HEUN == 2 ? 3 * R : 1];  // stage 2: x0, k1
   if constexpr (HEUN == 2) {
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      h0[3 * k] = ld(row(io.v0, rs + k), off);
-      h0[3 * k + 1] = ld(row(io.w0, rs + k), off);
-      h0[3 * k + 2] = ld(row(io.u0, rs + k), off);
-      hk[3 * k] = ld(row(io.k1v, rs + k), off);
-      hk[3 * k + 1] = ld(row(io.k1w, rs + k), off);
-      hk[3 * k + 2] = ld(row(io.k1u, rs + k), off);
+      h0[3 * k] = ldk(io.v0, k + 4);
+      h0[3 * k + 1] = ldk(io.w0, k + 4);
+      h0[3 * k + 2] = ldk(io.u0, k + 4);
+      hk[3 * k] = ldk(io.k1v, k + 4);
+      hk[3 * k + 1] = ldk(io.k1w, k + 4);
+      hk[3 * k + 2] = ldk(io.k1u, k + 4);
     }
   }
 
@@ -230,37 +242,44 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
       float sval = 0.0f;
       for (int m = 0; m < sv.count(); ++m) {
         if (!sv.on(m)) continue;
-        const float f = ld(row(sv.field(m), i), off);
+        const float f = ldk(sv.field(m), k + 4);
         sval = (f != 0.0f) ? f : sval;
       }
       jion = (sval != 0.0f) ? sval : jion;
     }
     const float du = del_u + jion;
     if (out_col && (!XG || i < H)) {  // rows past the grid only when H < R
-      const size_t ro = (size_t)i * (size_t)W;
       if constexpr (HEUN == 1) {  // x1 = x + dt k1, and k1
-        st(o_v + ro, off, fmaf(dt, dv, vv[k]));
-        st(o_w + ro, off, fmaf(dt, dw, ww[k]));
-        st(o_u + ro, off, fmaf(dt, du, uc));
-        st(io.kv + ro, off, dv);
-        st(io.kw + ro, off, dw);
-        st(io.ku + ro, off, du);
+        stk(o_v, k + 4, fmaf(dt, dv, vv[k]));
+        stk(o_w, k + 4, fmaf(dt, dw, ww[k]));
+        stk(o_u, k + 4, fmaf(dt, du, uc));
+        stk(io.kv, k + 4, dv);
+        stk(io.kw, k + 4, dw);
+        stk(io.ku, k + 4, du);
       } else if constexpr (HEUN == 2) {  // x0 + (k1 + k2) dt / 2
         const float half = dt * 0.5f;
-        st(o_v + ro, off, fmaf(hk[3 * k] + dv, half, h0[3 * k]));
-        st(o_w + ro, off, fmaf(hk[3 * k + 1] + dw, half, h0[3 * k + 1]));
-        st(o_u + ro, off, fmaf(hk[3 * k + 2] + du, half, h0[3 * k + 2]));
+        stk(o_v, k + 4, fmaf(hk[3 * k] + dv, half, h0[3 * k]));
+        stk(o_w, k + 4, fmaf(hk[3 * k + 1] + dw, half, h0[3 * k + 1]));
+        stk(o_u, k + 4, fmaf(hk[3 * k + 2] + du, half, h0[3 * k + 2]));
       } else if (EULER) {
-        st(o_v + ro, off, fmaf(dt, dv, vv[k]));
-        st(o_w + ro, off, fmaf(dt, dw, ww[k]));
-        st(o_u + ro, off, fmaf(dt, du, uc));
+        stk(o_v, k + 4, fmaf(dt, dv, vv[k]));
+        stk(o_w, k + 4, fmaf(dt, dw, ww[k]));
+        stk(o_u, k + 4, fmaf(dt, du, uc));
       } else {
-        st(o_v + ro, off, dv);
-        st(o_w + ro, off, dw);
-        st(o_u + ro, off, du);
+        stk(o_v, k + 4, dv);
+        stk(o_w, k + 4, dw);
+        stk(o_u, k + 4, du);
       }
     }
   }
+}
+
+// A band takes the edge path (clamped rows, one-sided rules) when it lies
+// within 4 rows of an edge, since the interior path reads rows rs-4 .. rs+R+3
+// unclamped, or when R + 8 rows span 4 GiB or more, past the interior path's
+// 32-bit row offsets.
+__host__ __device__ inline bool edge_band(int rs, int R, int H, int W) {
+  return rs < 4 || rs + R + 4 > H || (uint64_t)(R + 8) * (uint64_t)W * 4u > 0xffffffffull;
 }
 
 // One wave per (strip, band) tile, band-major within a strip so that
@@ -278,7 +297,7 @@ __device__ __forceinline__ void stream_kernel_body(
   const int strip = tile / g.n_bands, band = tile - strip * g.n_bands;
   const int cs = min(kStripOut * strip, g.cs_last);
   const int rs = min(R * band, g.rs_last);
-  const bool xg = rs < 3 || rs + R + 3 > H;          // band within 3 rows of an edge
+  const bool xg = edge_band(rs, R, H, W);
   const bool yg = cs < 3 || cs + kStripOut + 3 > W;  // strip within 3 columns
   const int sel = (xg ? 1 : 0) | (yg ? 2 : 0) | (sa.n ? 4 : 0);
   const StimListView sv{sa};

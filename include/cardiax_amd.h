@@ -69,7 +69,7 @@ int32_t fk_abi_version(void);
  * choice to XLA). Process-wide. Returns the previous setting, or FK_ERR_ARG
  * for an unknown variant.
  *    0      automatic: the autotuned choice for the grid shape, else 15 up to
- *           1.5 M cells and 13 above
+ *           1.5 M cells, 13 up to 8 M cells and 2 above
  *    1      plain, one thread per cell
  *    2, 3   LDS tile of 16 / 32 rows
  *    4-6    streaming wavefront bands of 8 / 16 / 32 rows
@@ -101,6 +101,12 @@ int32_t fk_autotune_lookup(int32_t H, int32_t W);
 
 /* Forgets every tuned shape. */
 void fk_autotune_reset(void);
+
+/* The variant fk_forward_euler runs on an H x W grid under the current
+ * setting: the fk_set_forward_variant choice when it is not 0, else the tuned
+ * one, else the default listed there; 17 / 18 on an odd W report the 15 / 13
+ * they run. No reference counterpart. FK_ERR_ARG for H or W below 3. */
+int32_t fk_forward_variant_for(int32_t H, int32_t W);
 
 /* cardiax/solve.py:18-23 -- v = w = 1, u = 0 over n_cells cells. */
 int32_t fk_init(float* v, float* w, float* u, int64_t n_cells, void* stream);

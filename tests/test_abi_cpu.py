@@ -44,6 +44,29 @@ def test_workspace_query():
     assert L.fk_forward_workspace_bytes(0, 5) == 0
 
 
+def test_default_variant_by_grid_size():
+    """Host logic only: which step kernel an automatic solve of a shape runs."""
+    from cardiax_amd import autotune
+
+    L = _lib.lib()
+    prev = L.fk_set_forward_variant(0)
+    try:
+        assert autotune.resolved((1024, 1024)) == "stream2"  # up to 1.5 M cells
+        assert autotune.resolved((1536, 1024)) == "stream2"
+        assert autotune.resolved((2048, 2048)) == "stream4"  # up to 8 M cells
+        assert autotune.resolved((2048, 4096)) == "stream4"
+        assert autotune.resolved((4096, 4096)) == "tile16"   # DRAM-resident grids
+        L.fk_set_forward_variant(_lib.VARIANTS["pair2"])
+        assert autotune.resolved((64, 64)) == "pair2"
+        assert autotune.resolved((64, 63)) == "stream2"      # odd W: the one-column kernel
+        L.fk_set_forward_variant(_lib.VARIANTS["pair4"])
+        assert autotune.resolved((64, 63)) == "stream4"
+        with pytest.raises(ValueError):
+            autotune.resolved((2, 64))
+    finally:
+        L.fk_set_forward_variant(prev)
+
+
 def _call_step(H=8, W=8, params=None, n_stim=0, stim=None, dx=0.01, null_D=False):
     L = _lib.lib()
     p = _lib.make_params(params or PARAMSET_3)
