@@ -133,8 +133,9 @@ k_batch_step(const SimDesc* __restrict__ sims, const TileItem* __restrict__ item
   const float dt = uni(S.dt), inv_dx2 = uni(S.inv_dx2);
   const int sel = item.flags | (mask ? 4 : 0);
 #define FK_BT(X, Y, ST)                                                                    \
-  stream_tile<R, true, X, Y, ST>(c, Dp, nullptr, i0, i1, i2, H, W, sv, dt, inv_dx2, o0, o1, o2, \
-                                       item.rs, item.cs)
+  stream_tile<R, true, X, Y, ST, BatchStimView, false, 0, true>(c, Dp, nullptr, i0, i1, i2, H, W,  \
+                                                                sv, dt, inv_dx2, o0, o1, o2,      \
+                                                                item.rs, item.cs)
   switch (sel) {
     case 0: FK_BT(false, false, false); break;
     case 1: FK_BT(true, false, false); break;

@@ -129,8 +129,11 @@ struct HeunIO {
 // PC: the static diffusion coefficients come precomputed from `cf`
 // (k_stream_coef) instead of being derived from D each step. HEUN: 0 = an
 // Euler step (EULER) or the derivatives, 1 / 2 = stage 1 / 2 of a Heun step.
+// FIELD_MAJOR: loads field by field with scalar row bases, measured faster
+// when the fields stream from DRAM (the batch); otherwise in the order the
+// output rows need them, measured faster for grids held in the 256 MB cache.
 template <int R, bool EULER, bool XG, bool YG, bool STIM, class SV, bool PC = false,
-          int HEUN = 0>
+          int HEUN = 0, bool FIELD_MAJOR = false>
 __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restrict__ D,
                                             const float4* __restrict__ cf,
                                             const float* __restrict__ v,
@@ -146,39 +149,71 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
   const uint32_t off = (uint32_t)clampi(col, 0, W - 1) * 4u;  // clamped = edge padding
   const bool out_col = lane >= 4 && lane < 4 + kStripOut && col < W;
   // Row rs - 4 + k of a field. Edge bands clamp the row (edge padding) and
-  // address it from a scalar row base. Interior bands (edge_band false) add
-  // one scalar band offset to each field base and a 32-bit per-row offset,
-  // k * 4W, that every field shares: a vector add per row instead of a scalar
-  // clamp, 64-bit multiply and add per row and field.
+  // address it from a scalar row base. Interior bands (edge_band false) need
+  // no clamp. With FIELD_MAJOR they also take scalar row bases (field base
+  // plus a row multiple); otherwise one scalar band offset per field base and
+  // a 32-bit per-row offset, k * 4W, that every field shares in a vector
+  // register.
   const uint32_t rb = (uint32_t)W * 4u;
   const size_t band = XG ? 0 : (size_t)(rs - 4) * (size_t)W * 4u;
   auto ldk = [&](const float* a, int k) {
-    if constexpr (XG) return ld(a + (size_t)clampi(rs - 4 + k, 0, H - 1) * (size_t)W, off);
-    else return ld((const float*)((const char*)a + band), off + (uint32_t)k * rb);
+    if constexpr (XG)
+      return ld(a + (size_t)clampi(rs - 4 + k, 0, H - 1) * (size_t)W, off);
+    else if constexpr (FIELD_MAJOR)
+      return ld(a + (size_t)(rs - 4 + k) * (size_t)W, off);
+    else
+      return ld((const float*)((const char*)a + band), off + (uint32_t)k * rb);
   };
   auto stk = [&](float* a, int k, float x) {  // rows inside the grid only
-    if constexpr (XG) st(a + (size_t)(rs - 4 + k) * (size_t)W, off, x);
-    else st((float*)((char*)a + band), off + (uint32_t)k * rb, x);
+    if constexpr (XG || FIELD_MAJOR)
+      st(a + (size_t)(rs - 4 + k) * (size_t)W, off, x);
+    else
+      st((float*)((char*)a + band), off + (uint32_t)k * rb, x);
   };
 
   float uu[R + 8];           // u rows rs-4 .. rs+R+3
   float dd[PC ? 1 : R + 4];  // D rows rs-2 .. rs+R+1
   float4 cc[PC ? R : 1];     // precomputed coefficients of rows rs .. rs+R-1
+  float vv[R], ww[R];
+  if constexpr (FIELD_MAJOR) {
+    // Field by field: every u row of the window, then D, then v and w.
 #pragma unroll
-  for (int k = 0; k < R + 8; ++k) uu[k] = ldk(u, k);
+    for (int k = 0; k < R + 8; ++k) uu[k] = ldk(u, k);
+    if constexpr (!PC) {
+#pragma unroll
+      for (int k = 0; k < R + 4; ++k) dd[k] = ldk(D, k + 2);
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      vv[k] = ldk(v, k + 4);
+      ww[k] = ldk(w, k + 4);
+    }
+  } else {
+    // In the order the output rows need them: first everything output row 0
+    // reads (u rows 0..8 of the window with D rows 2..6 alongside, then v and
+    // w of row 4), then the four rows each further output row adds. Loads
+    // return in order, so an output row's arithmetic starts once its own rows
+    // have arrived while the band's later rows are still in flight.
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      uu[k] = ldk(u, k);
+      if constexpr (!PC)
+        if (k >= 2 && k < 7) dd[k - 2] = ldk(D, k);
+    }
+    vv[0] = ldk(v, 4);
+    ww[0] = ldk(w, 4);
+#pragma unroll
+    for (int j = 1; j < R; ++j) {
+      uu[j + 8] = ldk(u, j + 8);
+      if constexpr (!PC) dd[j + 4] = ldk(D, j + 6);
+      vv[j] = ldk(v, j + 4);
+      ww[j] = ldk(w, j + 4);
+    }
+  }
   if constexpr (PC) {
 #pragma unroll
     for (int k = 0; k < R; ++k)
       cc[k] = ld4(cf + (size_t)clampi(rs + k, 0, H - 1) * (size_t)W, off * 4u);
-  } else {
-#pragma unroll
-    for (int k = 0; k < R + 4; ++k) dd[k] = ldk(D, k + 2);
-  }
-  float vv[R], ww[R];
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    vv[k] = ldk(v, k + 4);
-    ww[k] = ldk(w, k + 4);
   }
   float h0[HEUN == 2 ? 3 * R : 1], hk[HEUN == 2 ? 3 * R : 1];  // stage 2: x0, k1
   if constexpr (HEUN == 2) {
