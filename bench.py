@@ -228,6 +228,61 @@ def bench_inverse(seed, n=32, iters=50, n_steps=1000, frame_every=50):
             "d_err_final": round(float(np.mean(res["d_err_final"])), 4)}
 
 
+def bench_sequence(seed, n_steps, size=1200, every=500, reshape=256):
+    """deepx/generate.py:135-210 at the reference's production shape (1200^2
+    grid, a checkpoint every 500 steps, frames resized to 256^2,
+    experiments/generate_fd_data_256.py:7-12), over n_steps steps: the whole
+    generator loop (forward, on-GPU resize, copy of every frame to pinned host
+    memory; no file) against the same checkpointed forward alone."""
+    import numpy as np
+    import torch
+
+    from cardiax_amd import generate, io, solve, workloads
+    from cardiax_amd.stimulus import Protocol, Stimulus
+
+    sim = workloads.c2_inputs(seed, shape=(size, size), steps=n_steps)
+    dev = torch.device("cuda", 0)
+    stims = [Stimulus(Protocol(*s.protocol), torch.from_numpy(s.field).to(dev))
+             for s in sim.stimuli]
+    D = torch.from_numpy(sim.D).to(dev)
+    cps = list(range(0, n_steps, every))
+    from cardiax_amd import autotune
+
+    tuned = autotune.tune((size, size), n_steps=300)["best"]  # setup, as for the headline
+    generate.sequence(0, 3 * every, every, sim.dt, sim.dx, sim.params, D, stims, None,
+                      reshape=(reshape, reshape), plot_while=False)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    frames = generate.sequence(0, n_steps, every, sim.dt, sim.dx, sim.params, D, stims, None,
+                               reshape=(reshape, reshape), plot_while=False)
+    t_seq = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solve.forward(solve.init((size, size)), cps, sim.params, D, stims, sim.dt, sim.dx)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    # the resize kernel alone, one frame of 3 fields
+    x = torch.stack(list(st[-1]))
+    o = torch.empty((3, reshape, reshape), device=dev)
+    io.imresize(x, (reshape, reshape), out=o)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(50):
+        io.imresize(x, (reshape, reshape), out=o)
+    e1.record()
+    torch.cuda.synchronize()
+    cells = size * size * (cps[-1] - cps[0])
+    return {"workload": "%dx%d grid, %d Euler steps, a frame every %d steps resized to %dx%d "
+                        "and copied to pinned host memory (%d frames)"
+                        % (size, size, cps[-1], every, reshape, reshape, len(frames)),
+            "seconds": round(t_seq, 4), "gcups": round(cells / t_seq / 1e9, 3),
+            "forward_only_seconds": round(t_fwd, 4),
+            "forward_only_gcups": round(cells / t_fwd / 1e9, 3),
+            "overhead_frac": round(t_seq / t_fwd - 1.0, 4),
+            "resize_us_per_frame": round(e0.elapsed_time(e1) * 1e3 / 50, 2), "variant": tuned,
+            "frames_finite": bool(np.isfinite(frames).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,8 +304,13 @@ def main():
     ap.add_argument("--inverse-iters", type=int, default=50,
                     help="Adam iterations of the batched inverse problem (0 = skip it)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuner (the default kernel, stream8, runs)")
+                    help="skip the autotuner (the size-based default kernel runs)")
+    ap.add_argument("--sequence-steps", type=int, default=20_000,
+                    help="Euler steps of the dataset-generator leg (0 skips it)")
     args = ap.parse_args()
+    # stdout carries the one JSON line; the library's progress prints go to stderr
+    json_out = sys.stdout
+    sys.stdout = sys.stderr
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the solver has no multi-GPU path in this "
@@ -271,7 +331,8 @@ def main():
         for _ in range(20):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "bytes_written": 4 * n}))
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "bytes_written": 4 * n}),
+              file=json_out, flush=True)
         return
     H = W = args.size
     sim = workloads.c2_inputs(args.seed, shape=(H, W), steps=args.sim_steps)
@@ -354,6 +415,11 @@ def main():
         extra["inverse"] = bench_inverse(args.seed, iters=args.inverse_iters)
         log("inverse: %s" % json.dumps(extra["inverse"]))
 
+    if args.sequence_steps > 0:
+        log("dataset generator ...")
+        extra["sequence"] = bench_sequence(args.seed, args.sequence_steps)
+        log("sequence: %s" % json.dumps(extra["sequence"]))
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -382,7 +448,7 @@ def main():
         "cpu_baseline": cpu,
         "extra": extra,
     }
-    print(json.dumps(rec), flush=True)
+    print(json.dumps(rec), file=json_out, flush=True)
 
 
 if __name__ == "__main__":

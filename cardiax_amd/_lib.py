@@ -85,6 +85,8 @@ _SIGNATURES = {
                                       c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d, c_i32,
                                       c_f, c_f, c_vp, c_vp]),
     "fk_heun_workspace_bytes": (c_sz, [c_i32, c_i32]),
+    "fk_resize_workspace_bytes": (c_sz, [c_i32, c_i32, c_i32, c_i32, c_i32]),
+    "fk_resize": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_vp, c_sz, c_vp]),
     "fk_step_heun": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
                              ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_f, c_vp, c_vp,
                              c_vp, c_vp, c_sz, c_vp]),

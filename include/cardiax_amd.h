@@ -302,6 +302,23 @@ int32_t fk_batch_n_items(const fk_batch* b);
 /* Waits for the device and releases the batch. */
 int32_t fk_batch_destroy(fk_batch* b);
 
+/* ---------------------------------------------------------------------------
+ * Checkpoint output of the dataset generator (deepx/generate.py:135-210).
+ * ------------------------------------------------------------------------- */
+
+/* Workspace of fk_resize: n_images * h * W doubles (the row pass's result). */
+size_t fk_resize_workspace_bytes(int32_t n_images, int32_t H, int32_t W, int32_t h, int32_t w);
+
+/* cardiax/io.py:118-124 -- imresize(a, (h, w)) = jax.image.resize(a,
+ * a.shape[:-2] + (h, w), "bilinear"), which is antialiased by default: n_images
+ * contiguous H x W float32 images in `src` resampled to h x w in `dst`, along
+ * each axis with a triangle kernel widened by the downsampling factor and
+ * weights normalised to 1 (an unchanged axis is copied exactly). Sizes >= 1;
+ * dst must not overlap src; work holds fk_resize_workspace_bytes. Stream-
+ * ordered. */
+int32_t fk_resize(const float* src, int32_t n_images, int32_t H, int32_t W, float* dst,
+                  int32_t h, int32_t w, void* work, size_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

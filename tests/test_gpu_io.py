@@ -1,0 +1,105 @@
+"""GPU tests of the dataset generator's output path: fk_resize against the
+FP64 restatement of imresize (cardiax/io.py:118-124), and sequence
+(deepx/generate.py:135-210) against its own pieces (bitwise) and the FP64
+oracle, with the file layout checked through a dict-backed h5py stand-in."""
+import numpy as np
+import pytest
+import torch
+
+from cardiax_amd import generate, solve
+from cardiax_amd import io as cio
+from cardiax_amd import workloads as wl
+from cardiax_amd.params import PARAMSET_3
+from cardiax_amd.stimulus import Protocol, Stimulus
+from oracle import c_oracle, io_oracle
+from oracle import fk_oracle as o
+
+import fake_h5py
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(600, 600, 128, 128), (1200, 1200, 256, 256), (37, 53, 11, 7), (10, 10, 4, 4),
+          (8, 8, 16, 16), (13, 9, 13, 20), (64, 64, 1, 1), (1, 9, 1, 4), (7, 7, 21, 3),
+          (1024, 1024, 1024, 1024)]
+
+
+@pytest.mark.parametrize("H,W,h,w", SHAPES)
+def test_resize_matches_oracle(gpu, H, W, h, w):
+    rng = np.random.default_rng(H + 7 * W)
+    a = rng.standard_normal((3, H, W)).astype(np.float32)
+    got = cio.imresize(torch.from_numpy(a).cuda(), (h, w)).cpu().numpy()
+    ref = io_oracle.imresize(a.astype(np.float64), (h, w))
+    assert got.shape == (3, h, w) and got.dtype == np.float32
+    # the kernel sums in double and rounds once: within one float32 ulp of the FP64 result
+    np.testing.assert_allclose(got, ref, rtol=2 ** -23, atol=1e-12)
+    if (H, W) == (h, w):
+        assert np.array_equal(got, a)  # unchanged axes are copied exactly
+
+
+def test_resize_leading_axes_and_reuse(gpu):
+    rng = np.random.default_rng(5)
+    a = torch.from_numpy(rng.standard_normal((2, 3, 40, 30)).astype(np.float32)).cuda()
+    out = torch.empty((2, 3, 16, 12), device="cuda")
+    r = cio.imresize(a, (16, 12), out=out)
+    assert r is out
+    for i in range(2):
+        for k in range(3):
+            one = cio.imresize(a[i, k], (16, 12))
+            assert torch.equal(one, out[i, k])
+    with pytest.raises(ValueError):
+        cio.imresize(a, (16, 12), out=torch.empty((2, 3, 16, 11), device="cuda"))
+    with pytest.raises(ValueError):
+        cio.imresize(a[0, 0, 0], (2, 2))
+
+
+def _small_run(seed=2, shape=(64, 64)):
+    sim = wl.c1_inputs(seed, shape=shape, steps=300)
+    stims = [Stimulus(Protocol(*s.protocol), s.field) for s in sim.stimuli]
+    return sim, stims
+
+
+@pytest.mark.parametrize("reshape", [(32, 32), None, (48, 40)])
+def test_sequence_frames_are_the_resized_checkpoints(gpu, reshape):
+    sim, stims = _small_run()
+    frames = generate.sequence(0, 300, 50, 0.01, 0.01, sim.params, sim.D, stims, None,
+                               reshape=reshape, plot_while=False)
+    out_shape = reshape or sim.D.shape
+    assert frames.shape == (5, 3) + tuple(out_shape)
+    # bitwise: the same forward segments and resizes called one by one
+    st = solve.init(sim.D.shape)
+    for i, (a, b) in enumerate(zip(range(0, 250, 50), range(50, 300, 50))):
+        st = solve._forward_euler(st, a, b, sim.params, sim.D, stims, 0.01, 0.01)
+        fr = cio.imresize(torch.stack(list(st)), out_shape).cpu().numpy()
+        assert np.array_equal(frames[i], fr), i
+    # FP64 oracle: forward and resize, within the parity bar
+    ost = [o.Stimulus(o.Protocol(*s.protocol), s.field) for s in sim.stimuli]
+    ref = io_oracle.frames(0, 300, 50, 0.01, 0.01, sim.params, sim.D, ost, reshape,
+                           forward_euler=c_oracle.forward_euler)
+    assert np.abs(frames - ref).max() <= 1e-4
+    assert frames[-1, 2].max() > 0.5  # waves under way
+
+
+@pytest.mark.parametrize("use_memory", [True, False])
+def test_sequence_file_layout(gpu, tmp_path, monkeypatch, use_memory):
+    monkeypatch.setattr(cio, "_h5py", lambda: fake_h5py)
+    sim, stims = _small_run(3, (48, 56))
+    path = str(tmp_path / "seq.hdf5")
+    ret = generate.sequence(0, 250, 50, 0.01, 0.01, sim.params, sim.D, stims, path,
+                            reshape=(24, 28), use_memory=use_memory, plot_while=False)
+    frames = generate.sequence(0, 250, 50, 0.01, 0.01, sim.params, sim.D, stims, None,
+                               reshape=(24, 28), plot_while=False)
+    assert (ret is None) == (not use_memory)
+    assert fake_h5py.File.files[path].closed  # the generator closes its file
+    flat = fake_h5py.File(path, "r").flat()
+    lay = io_oracle.layout(5, len(stims), (24, 28), PARAMSET_3._fields)
+    assert set(flat) == set(lay)
+    for k, shp in lay.items():
+        assert np.shape(flat[k]) == shp, k
+    assert np.array_equal(flat["states"][:4], frames)  # 4 intervals, written either way
+    assert not flat["states"][4].any()                 # the reference's unwritten frame
+    Dr = cio.imresize(sim.D, (24, 28)).cpu().numpy()
+    assert np.array_equal(flat["params/D"], Dr) and np.array_equal(flat["diffusivity"], Dr)
+    for i, s in enumerate(stims):
+        assert np.array_equal(flat["field"][i], cio.imresize(s.field, (24, 28)).cpu().numpy())
+        assert float(flat["start"][i]) == float(s.protocol.start)
+    assert float(flat["params/tau_d"]) == pytest.approx(float(sim.params.tau_d))
