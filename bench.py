@@ -90,7 +90,20 @@ def cpu_baseline(sim, target_s=12.0):
     return {"value": H * W * n1 / el / 1e9, "unit": "GCUPS", "cores": cores, "kind": "port",
             "sample": "%dx%d grid, Euler steps %d..%d of the same inputs, fp32 C "
                       "restatement of the reference op graph, %d OpenMP threads, %.1f s"
-                      % (H, W, n0, n0 + n1, cores, el)}
+                      % (H, W, n0, n0 + n1, cores, el),
+            "cpu_model": cpu_model()}
+
+
+def cpu_model():
+    """Model name of the host CPU (/proc/cpuinfo), as lscpu reports it."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def bench_batched(n_sims, n_steps, seed, per_sim_steps=64):

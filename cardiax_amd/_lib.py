@@ -87,6 +87,9 @@ _SIGNATURES = {
     "fk_heun_workspace_bytes": (c_sz, [c_i32, c_i32]),
     "fk_resize_workspace_bytes": (c_sz, [c_i32, c_i32, c_i32, c_i32, c_i32]),
     "fk_resize": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_vp, c_sz, c_vp]),
+    "fk_stimulus_box": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_f, c_vp]),
+    "fk_affine_transform": (c_i32, [c_vp, c_i32, c_i32, ctypes.POINTER(c_d),
+                                    ctypes.POINTER(c_d), c_vp, c_vp]),
     "fk_step_heun": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
                              ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_f, c_vp, c_vp,
                              c_vp, c_vp, c_sz, c_vp]),

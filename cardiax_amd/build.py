@@ -17,7 +17,8 @@ INCLUDE = os.path.join(ROOT, "include")
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcardiax_amd.so")
 
-SOURCES = ["fk_common.hip", "fk_forward.hip", "fk_batch.hip", "fk_adjoint.hip", "fk_io.hip"]
+SOURCES = ["fk_common.hip", "fk_forward.hip", "fk_batch.hip", "fk_adjoint.hip", "fk_io.hip",
+           "fk_stimulus.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

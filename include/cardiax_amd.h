@@ -319,6 +319,32 @@ size_t fk_resize_workspace_bytes(int32_t n_images, int32_t H, int32_t W, int32_t
 int32_t fk_resize(const float* src, int32_t n_images, int32_t H, int32_t W, float* dst,
                   int32_t h, int32_t w, void* work, size_t work_bytes, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Stimulus fields built on the device (cardiax/stimulus.py:31-140).
+ * ------------------------------------------------------------------------- */
+
+/* cardiax/stimulus.py:31-60 (rectangular) and :63-106 (linear) -- field =
+ * modulus on rows [r0, r1) x columns [c0, c1), 0 elsewhere. The bounds are
+ * the reference's slices already normalised by Python slice semantics:
+ * 0 <= r0, r1 <= H, 0 <= c0, c1 <= W (an empty range gives an all-zero
+ * field). */
+int32_t fk_stimulus_box(float* field, int32_t H, int32_t W, int32_t r0, int32_t r1, int32_t c0,
+                        int32_t c1, float modulus, void* stream);
+
+/* cardiax/stimulus.py:137-139 (triangular) -- the resampling that
+ * scipy.ndimage.rotate(src, angle, mode="nearest", prefilter=False,
+ * reshape=False) performs: scipy.ndimage.affine_transform with order 3, no
+ * prefilter and mode "nearest". Output cell o = (row, col) of the H x W dst
+ * samples the cubic B-spline of the unfiltered src at input coordinate
+ * matrix . o + offset (matrix: 4 doubles, row-major; offset: 2 doubles; both
+ * host memory read before the call returns); taps outside the grid take the
+ * nearest edge value. The caller builds the map as scipy's rotate does:
+ * matrix = [[cos a, sin a], [-sin a, cos a]] (scipy.special.cosdg/sindg) and
+ * offset = c - matrix @ c, c = ((H-1)/2, (W-1)/2). dst must differ from src.
+ * Stream-ordered. */
+int32_t fk_affine_transform(const float* src, int32_t H, int32_t W, const double* matrix,
+                            const double* offset, float* dst, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
