@@ -21,6 +21,7 @@ FK_OK = 0
 FK_ERR_ARG = -1
 FK_ERR_HIP = -2
 FK_ERR_UNSUPPORTED = -3
+FK_ERR_NONFINITE = -4
 FK_MAX_STIMULI = 32
 
 # Forward kernel variants (fk_set_forward_variant).
@@ -69,6 +70,7 @@ _SIGNATURES = {
     "fk_forward_variant_for": (c_i32, [c_i32, c_i32]),
     "fk_autotune_reset": (None, []),
     "fk_init": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "fk_check_finite": (c_i32, [ctypes.POINTER(c_vp), c_i32, c_i64, c_vp, c_vp]),
     "fk_step": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
                         ctypes.POINTER(FkStimulus), c_i32, c_d, c_f, c_vp, c_vp, c_vp, c_vp]),
     "fk_step_euler": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32, c_i32,
@@ -151,6 +153,8 @@ def check(rc: int) -> None:
         raise ValueError(msg)
     if rc == FK_ERR_UNSUPPORTED:
         raise NotImplementedError(msg)
+    if rc == FK_ERR_NONFINITE:
+        raise FloatingPointError(msg)
     raise RuntimeError("HIP error: " + msg)
 
 

@@ -245,6 +245,22 @@ __global__ void k_init(float* v, float* w, float* u, int64_t n) {
   }
 }
 
+// Sets *flag when any of the n values is NaN or infinite. The flag word is
+// cleared by k_clear_flag first (a kernel rather than a memset, so the pair
+// can sit in a captured graph).
+__global__ void k_clear_flag(unsigned* flag) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 0u;
+}
+
+__global__ void __launch_bounds__(256)
+k_nonfinite(const float* __restrict__ a, int64_t n, unsigned* flag) {
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    bad |= !isfinite(a[i]);
+  if (bad) atomicOr(flag, 1u);
+}
+
 // cardiax/solve.py:225-254 on an unpadded (rows, cols) array.
 __global__ void k_gradient(const float* __restrict__ a, float* __restrict__ out, int rows,
                            int cols, int axis) {
@@ -618,6 +634,27 @@ int32_t fk_init(float* v, float* w, float* u, int64_t n_cells, void* stream) {
   const int blocks = (int)std::min<int64_t>((n_cells + 255) / 256, 4096);
   k_init<<<blocks, 256, 0, (hipStream_t)stream>>>(v, w, u, n_cells);
   HIP_CHECK(hipGetLastError());
+  return FK_OK;
+}
+
+int32_t fk_check_finite(const float* const* fields, int32_t n_fields, int64_t n_cells,
+                        void* work, void* stream) {
+  ErrorScope es;
+  if (!fields || !work || n_fields < 1 || n_cells < 0)
+    return set_error(FK_ERR_ARG, "fk_check_finite: null pointer or bad count");
+  for (int32_t k = 0; k < n_fields; ++k)
+    if (!fields[k]) return set_error(FK_ERR_ARG, "fk_check_finite: field %d is null", k);
+  hipStream_t s = (hipStream_t)stream;
+  unsigned* flag = (unsigned*)work;
+  k_clear_flag<<<1, 64, 0, s>>>(flag);
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n_cells + 255) / 256, 2048));
+  for (int32_t k = 0; k < n_fields; ++k)
+    k_nonfinite<<<blocks, 256, 0, s>>>(fields[k], n_cells, flag);
+  HIP_CHECK(hipGetLastError());
+  unsigned host = 0;
+  HIP_CHECK(hipMemcpyAsync(&host, flag, sizeof(host), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (host) return set_error(FK_ERR_NONFINITE, "non-finite values (NaN or inf) in the fields");
   return FK_OK;
 }
 

@@ -265,6 +265,21 @@ def forward_dimensional(tissue_size, final_time, ms_step, params, diffusivity, s
                    plot_while)
 
 
+def check_finite(state) -> None:
+    """Raise FloatingPointError when any field of ``state`` (on the GPU)
+    holds NaN or inf (fk_check_finite; synchronises the stream). No
+    reference counterpart: JAX carries NaN through silently."""
+    dev = require_gpu("check_finite")
+    fields = [on_device(x, dev, "field") for x in state]
+    n = fields[0].numel()
+    if any(f.numel() != n for f in fields):
+        raise ValueError("check_finite: fields differ in size")
+    ptrs = (ctypes.c_void_p * len(fields))(*[f.data_ptr() for f in fields])
+    flag = torch.empty(1, dtype=torch.int32, device=dev)
+    _lib.check(_lib.lib().fk_check_finite(ptrs, len(fields), n, flag.data_ptr(),
+                                          stream_handle()))
+
+
 def gradient(a, axis):
     """cardiax/solve.py:225-254: first derivative of a 2-D array along
     ``axis`` without spacing (4th-order central inside, 3rd-order one-sided on

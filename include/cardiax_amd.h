@@ -36,7 +36,8 @@ enum {
   FK_OK = 0,
   FK_ERR_ARG = -1,         /* invalid argument: shape, pointer, count, range */
   FK_ERR_HIP = -2,         /* a HIP runtime call failed */
-  FK_ERR_UNSUPPORTED = -3  /* valid request outside this build's limits */
+  FK_ERR_UNSUPPORTED = -3, /* valid request outside this build's limits */
+  FK_ERR_NONFINITE = -4    /* fk_check_finite found NaN or inf in device data */
 };
 
 /* Most stimuli one call accepts (the reference takes any list length). */
@@ -110,6 +111,15 @@ int32_t fk_forward_variant_for(int32_t H, int32_t W);
 
 /* cardiax/solve.py:18-23 -- v = w = 1, u = 0 over n_cells cells. */
 int32_t fk_init(float* v, float* w, float* u, int64_t n_cells, void* stream);
+
+/* The device-side non-finite check: FK_OK when every value of the n_fields
+ * device fields (fields: a host array of device pointers, n_cells floats each)
+ * is finite, FK_ERR_NONFINITE when any is NaN or infinite. work: at least 4
+ * bytes of device memory for the flag. Enqueued on `stream`, which it then
+ * synchronises to read the flag back. No reference counterpart: JAX carries
+ * NaN through silently; a caller checks a state after a solve with this. */
+int32_t fk_check_finite(const float* const* fields, int32_t n_fields, int64_t n_cells,
+                        void* work, void* stream);
 
 /* cardiax/solve.py:26-65 -- the time derivatives (d_v, d_w, d_u) of one
  * state at time t: edge-padded fields, gates and ionic currents, stimulus

@@ -104,6 +104,20 @@ def test_argument_errors_are_reported_before_any_device_work():
     assert rc == _lib.FK_ERR_ARG and b"workspace" in L.fk_last_error()
 
 
+def test_check_finite_arguments():
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    ptrs = (ctypes.c_void_p * 2)(0x1000, None)
+    assert L.fk_check_finite(ptrs, 2, 10, fake, None) == _lib.FK_ERR_ARG
+    assert b"null" in L.fk_last_error()
+    assert L.fk_check_finite(ptrs, 0, 10, fake, None) == _lib.FK_ERR_ARG
+    assert L.fk_check_finite(None, 1, 10, fake, None) == _lib.FK_ERR_ARG
+    assert L.fk_check_finite(ptrs, 1, -1, fake, None) == _lib.FK_ERR_ARG
+    assert L.fk_check_finite(ptrs, 1, 10, None, None) == _lib.FK_ERR_ARG
+    with pytest.raises(FloatingPointError):
+        _lib.check(_lib.FK_ERR_NONFINITE)
+
+
 def test_check_maps_codes_to_python_errors():
     _call_step(H=1)
     with pytest.raises(ValueError):
