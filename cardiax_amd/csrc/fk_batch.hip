@@ -187,7 +187,7 @@ struct fk_batch {
 
 static int32_t batch_check_sim(const fk_sim& s, int i) {
   if (!check_params(&s.params) || !check_grid(s.H, s.W) ||
-      !check_ptrs({s.D, s.v, s.w, s.u}) || !check_scalar("t", s.t) ||
+      !check_ptrs({s.D, s.v, s.w, s.u}) || !check_time(s.t, s.t) ||
       !check_scalar("dt", s.dt) || !check_dx(s.dx))
     return set_error(last_code(), "simulation %d: %s", i, fk_last_error());
   if (s.n_stimuli < 0 || s.n_stimuli > FK_BATCH_MAX_STIMULI)
@@ -418,6 +418,8 @@ int32_t fk_batch_run(fk_batch* b, int64_t n_steps, int32_t use_graph, void* stre
   ErrorScope es;
   if (b == nullptr) return set_error(FK_ERR_ARG, "batch is null");
   if (n_steps < 0) return set_error(FK_ERR_ARG, "negative step count");
+  for (int i = 0; i < b->n_sims; ++i)
+    if (!check_time(b->host_sims[i].t, b->host_sims[i].t + (double)n_steps)) return last_code();
   hipStream_t s = (hipStream_t)stream;
   // state in: caller buffers -> pool 0
   for (int i = 0; i < b->n_sims; ++i) {

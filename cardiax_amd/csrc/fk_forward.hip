@@ -664,7 +664,7 @@ int32_t fk_step(const fk_params* params, const float* D, const float* v, const f
                 void* stream) {
   ErrorScope es;
   if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
-      !check_ptrs({D, v, w, u, dv, dw, du}) || !check_scalar("t", t) || !check_dx(dx))
+      !check_ptrs({D, v, w, u, dv, dw, du}) || !check_time(t, t) || !check_dx(dx))
     return last_code();
   StimSchedule sched(stimuli, n_stimuli);
   const Coef c = make_coef(*params);
@@ -680,7 +680,7 @@ int32_t fk_step_euler(const fk_params* params, const float* D, const float* v,
                       float dx, float* v_out, float* w_out, float* u_out, void* stream) {
   ErrorScope es;
   if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
-      !check_ptrs({D, v, w, u, v_out, w_out, u_out}) || !check_scalar("t", t) ||
+      !check_ptrs({D, v, w, u, v_out, w_out, u_out}) || !check_time(t, t) ||
       !check_scalar("dt", dt) || !check_dx(dx))
     return last_code();
   StimSchedule sched(stimuli, n_stimuli);
@@ -836,7 +836,7 @@ int32_t fk_step_heun(const fk_params* params, const float* D, const float* v, co
                      float* w_out, float* u_out, void* work, size_t work_bytes, void* stream) {
   ErrorScope es;
   if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
-      !check_ptrs({D, v, w, u, v_out, w_out, u_out, work}) || !check_scalar("t", t) ||
+      !check_ptrs({D, v, w, u, v_out, w_out, u_out, work}) || !check_time(t, t) ||
       !check_scalar("dt", dt) || !check_dx(dx))
     return last_code();
   if (work_bytes < fk_heun_workspace_bytes(H, W))
@@ -895,7 +895,7 @@ int32_t fk_stimulate(double t, const float* X, float* out, int64_t n_cells,
                      const fk_stimulus* stimuli, int32_t n_stimuli, void* stream) {
   ErrorScope es;
   if (n_cells < 0) return set_error(FK_ERR_ARG, "fk_stimulate: negative size");
-  if (!check_ptrs({X, out}) || !check_stimuli(stimuli, n_stimuli) || !check_scalar("t", t))
+  if (!check_ptrs({X, out}) || !check_stimuli(stimuli, n_stimuli) || !check_time(t, t))
     return last_code();
   if (n_cells == 0) return FK_OK;
   StimSchedule sched(stimuli, n_stimuli);

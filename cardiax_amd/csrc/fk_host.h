@@ -129,13 +129,20 @@ inline int64_t loop_count(double t, double t_end) {
   return n > 0 ? (int64_t)n : 0;
 }
 
+// Bound on time indices (step units): 2^24.
+constexpr double kMaxTime = 16777216.0;
+
 inline bool check_time(double t, double t_end) {
   if (!std::isfinite(t) || !std::isfinite(t_end)) {
     set_error(FK_ERR_ARG, "t and t_end must be finite");
     return false;
   }
-  if (t_end - t > 4.0e9) {
-    set_error(FK_ERR_ARG, "too many steps in one call (%.0f)", t_end - t);
+  // The reference's step counter is float32 when reached through forward()
+  // and int32 through the dataset generator; the two agree on every step
+  // index below 2^24 (a loop may end at 2^24).
+  if (std::fabs(t) >= kMaxTime || std::fabs(t_end) > kMaxTime) {
+    set_error(FK_ERR_ARG, "time indices must stay below 2^24 (t %.17g, t_end %.17g)", t,
+              t_end);
     return false;
   }
   return true;

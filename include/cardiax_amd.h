@@ -14,7 +14,12 @@
  *   - All array pointers are device pointers; structs and descriptor arrays
  *     are host memory read before the call returns.
  *   - Work is enqueued on `stream` (a hipStream_t; NULL = the null stream) and
- *     is asynchronous: no entry point synchronises the stream.
+ *     is asynchronous. Only the calls documented to wait synchronise:
+ *     fk_check_finite (reads its flag back), fk_autotune_forward,
+ *     fk_batch_autotune, fk_batch_set_rows and fk_batch_destroy.
+ *   - Step indices must lie below 2^24 in magnitude (a loop may end at 2^24),
+ *     where the reference's float32 and int32 step counters agree
+ *     (FK_ERR_ARG beyond).
  *   - No entry point allocates device memory except fk_batch_create, which
  *     owns the device copy of the batch work list until fk_batch_destroy,
  *     and fk_autotune_forward, which frees its timing scratch before it

@@ -104,6 +104,23 @@ def test_argument_errors_are_reported_before_any_device_work():
     assert rc == _lib.FK_ERR_ARG and b"workspace" in L.fk_last_error()
 
 
+def test_time_indices_bounded_by_float32_counter():
+    L = _lib.lib()
+    p = _lib.make_params(PARAMSET_3)
+    fake = ctypes.c_void_p(0x1000)
+
+    def fwd(t, t_end):
+        return L.fk_forward_euler(ctypes.byref(p), fake, fake, fake, fake, 8, 8, None, 0, t,
+                                  t_end, 0.01, 0.01, fake, fake, fake, fake, 1 << 20, None)
+
+    assert fwd(0.0, 2.0 ** 24 + 1) == _lib.FK_ERR_ARG
+    assert b"2^24" in L.fk_last_error()
+    assert fwd(2.0 ** 24, 2.0 ** 24 + 5) == _lib.FK_ERR_ARG
+    assert fwd(-2.0 ** 24, 0.0) == _lib.FK_ERR_ARG
+    assert L.fk_step(ctypes.byref(p), fake, fake, fake, fake, 8, 8, None, 0, 3e7, 0.01, fake,
+                     fake, fake, None) == _lib.FK_ERR_ARG
+
+
 def test_check_finite_arguments():
     L = _lib.lib()
     fake = ctypes.c_void_p(0x1000)
