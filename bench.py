@@ -241,6 +241,36 @@ def bench_inverse(seed, n=32, iters=50, n_steps=1000, frame_every=50):
             "d_err_final": round(float(np.mean(res["d_err_final"])), 4)}
 
 
+def bench_heun(seed, size=1024, n_steps=2000):
+    """TimeIntegrator.HEUN (cardiax/solve.py:103-111) on the single-tissue
+    workload: two streaming launches per step. Algorithmic bytes per Heun
+    cell-step: stage 1 reads v, w, u, D and writes x1 and k1 (40 B); stage 2
+    reads x1, D, x0 and k1 and writes the state (52 B)."""
+    import torch
+
+    from cardiax_amd import solve, workloads
+
+    sim = workloads.c2_inputs(seed, shape=(size, size), steps=n_steps)
+    dev = torch.device("cuda", 0)
+    stims = [(s.protocol, torch.from_numpy(s.field).to(dev)) for s in sim.stimuli]
+    D = torch.from_numpy(sim.D).to(dev)
+    st0 = solve.init((size, size))
+    solve._forward_heun(st0, 0, 200, sim.params, D, stims, sim.dt, sim.dx)  # warm-up
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    out = solve._forward_heun(st0, 0, n_steps, sim.params, D, stims, sim.dt, sim.dx)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3
+    gcups = size * size * n_steps / sec / 1e9
+    return {"workload": "%dx%d grid, %d Heun steps (two launches each)" % (size, size, n_steps),
+            "gcups": round(gcups, 3), "us_per_step": round(sec / n_steps * 1e6, 3),
+            "hbm_frac": round(gcups * 92 / HBM_PEAK_GBS, 4),
+            "u_finite": bool(torch.isfinite(out.u).all().item())}
+
+
 def bench_sequence(seed, n_steps, size=1200, every=500, reshape=256):
     """deepx/generate.py:135-210 at the reference's production shape (1200^2
     grid, a checkpoint every 500 steps, frames resized to 256^2,
@@ -427,6 +457,10 @@ def main():
         log("inverse problem ...")
         extra["inverse"] = bench_inverse(args.seed, iters=args.inverse_iters)
         log("inverse: %s" % json.dumps(extra["inverse"]))
+
+    log("heun ...")
+    extra["heun"] = bench_heun(args.seed)
+    log("heun: %s" % json.dumps(extra["heun"]))
 
     if args.sequence_steps > 0:
         log("dataset generator ...")
