@@ -1,5 +1,19 @@
-"""Mirror of deepx.generate.sequence (deepx/generate.py:135-210): the dataset
-generator, the production caller of the forward-Euler path.
+"""Mirror of deepx/generate.py: the dataset generator, the production caller
+of the forward-Euler path (``sequence``, deepx/generate.py:135-210), and the
+random inputs ``random_sequence`` draws for it (:16-132).
+
+The reference draws from ``jax.random`` keys, which are absent here, so the
+random draws are project-defined: a seed (or NumPy Generator or
+SeedSequence) takes the place of the key and ``SeedSequence.spawn`` the place
+of ``jax.random.split``. The distributions, ranges and quirks are the
+reference's: the rectangle size and centre both follow ``shape[0]``, the
+linear and triangular directions are drawn from {NORTH, EAST, SOUTH} (never
+WEST), the coverage and angle come from the same key as the direction,
+``random_sequence`` reuses the last stimulus key for the D-map, and an
+empty integer range returns its lower bound as ``jax.random.randint`` does.
+The D-map comes from the seeded scar generator of ``workloads.scar_map``
+(the reference's spline scars, deepx/utils_scars.py:402-426, need its PRNG)
+rescaled onto ``domain`` as the reference rescales its own.
 
 The reference runs ``_forward_euler`` per checkpoint interval, pulls each
 state to the host and resizes it there with ``jax.image.resize`` before
@@ -14,8 +28,104 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import convert, io, solve
+from . import convert, io, solve, stimulus, workloads
 from ._device import require_gpu
+
+
+def _seq(rng) -> np.random.SeedSequence:
+    if isinstance(rng, np.random.SeedSequence):
+        return rng
+    if isinstance(rng, np.random.Generator):
+        return np.random.SeedSequence(int(rng.integers(0, 2 ** 63)))
+    return np.random.SeedSequence(int(rng))
+
+
+def _split(rng, n):
+    """jax.random.split: n independent child generators."""
+    return [np.random.default_rng(c) for c in _seq(rng).spawn(n)]
+
+
+def _randint(g, lo, hi, size=None):
+    """jax.random.randint(key, size, lo, hi): [lo, hi), lo when hi <= lo."""
+    lo, hi = int(lo), int(hi)
+    if hi <= lo:
+        return lo if size is None else np.full(size, lo, dtype=np.int64)
+    return int(g.integers(lo, hi)) if size is None else g.integers(lo, hi, size=size)
+
+
+def random_protocol(rng, min_start=0, max_start=1000, min_period=400, max_period=1e9):
+    """deepx/generate.py:16-27 -- start in [min_start, max_start), duration 2,
+    period in [min_period, max_period)."""
+    g1, g2 = _split(rng, 2)
+    return stimulus.Protocol(_randint(g1, min_start, max_start), 2,
+                             _randint(g2, min_period, max_period))
+
+
+def random_rectangular_stimulus(rng, shape, protocol, modulus=0.6):
+    """deepx/generate.py:30-36 -- sizes in [max(shape[0] // 100, 10),
+    shape[0] // 3), centre in [min(size), shape[0]) on both axes."""
+    g1, g2 = _split(rng, 2)
+    size = _randint(g2, max(shape[0] // 100, 10), shape[0] // 3, size=2)
+    centre = _randint(g1, int(np.min(size)), shape[0], size=2)
+    return stimulus.rectangular(shape, tuple(int(x) for x in centre),
+                                tuple(int(x) for x in size), modulus, protocol)
+
+
+def random_linear_stimulus(rng, shape, protocol, modulus=0.6):
+    """deepx/generate.py:39-45 -- coverage |N(0, 1)| * 0.2, direction in
+    {NORTH, EAST, SOUTH}."""
+    g1, _ = _split(rng, 2)
+    coverage = abs(float(g1.normal()))
+    direction = stimulus.Direction(_randint(g1, 0, 3))
+    return stimulus.linear(shape, direction, coverage * 0.2, modulus, protocol)
+
+
+def random_triangular_stimulus(rng, shape, protocol, modulus=0.6):
+    """deepx/generate.py:48-56 -- angle |N(0, 1)| * 45 degrees, coverage
+    |N(0, 1)| * 0.2, direction in {NORTH, EAST, SOUTH}."""
+    g1, _ = _split(rng, 2)
+    angle, coverage = np.abs(g1.normal(size=2))
+    direction = stimulus.Direction(_randint(g1, 0, 3))
+    return stimulus.triangular(shape, direction, float(angle) * 45, float(coverage) * 0.2,
+                               modulus, protocol)
+
+
+def random_stimulus(rng, shape, min_start=0, max_start=0):
+    """deepx/generate.py:59-76 -- a random protocol and one of the
+    rectangular, triangular and linear builders (uniformly), modulus 20."""
+    g1, g2, g3 = _split(rng, 3)
+    protocol = random_protocol(g1, min_start=min_start, max_start=max_start)
+    builders = (random_rectangular_stimulus, random_triangular_stimulus, random_linear_stimulus)
+    return builders[_randint(g2, 0, len(builders))](g3, shape, protocol, 20.0)
+
+
+def random_diffusivity(rng, shape, domain=(0.0001, 0.001)):
+    """deepx/generate.py:79-83 -- a scar map rescaled onto ``domain``
+    (cardiax/convert.py:23-26): its minimum becomes domain[0], its maximum
+    domain[1]."""
+    g = rng if isinstance(rng, np.random.Generator) else np.random.default_rng(_seq(rng))
+    c = workloads.scar_map(tuple(shape), g).astype(np.float64)
+    return convert.diffusivity_rescale(c, domain).astype(np.float32)
+
+
+def random_sequence(rng, params, filepath, shape=(1200, 1200), n_stimuli=2, start=0,
+                    stop=1000, step=1, dt=0.01, dx=0.01, reshape=None, use_memory=False,
+                    plot_while=True):
+    """deepx/generate.py:86-132 -- n_stimuli random stimuli, the i-th starting
+    at step 1 + i * ms_to_units(tau_d * 1000, dt), a random D-map, and
+    ``sequence`` over [start, stop) ms with a checkpoint every ``step`` ms."""
+    seqs = _seq(rng).spawn(n_stimuli)
+    max_start = np.arange(1, convert.ms_to_units(stop, dt),
+                          convert.ms_to_units(params.tau_d * 1000, dt))
+    stimuli = [random_stimulus(np.random.default_rng(seqs[i]), shape,
+                               min_start=max_start[i], max_start=max_start[i] + 1)
+               for i in range(n_stimuli)]
+    # the reference draws the D-map from the last stimulus key again
+    diffusivity = random_diffusivity(np.random.default_rng(seqs[-1]), shape)
+    return sequence(start=convert.ms_to_units(start, dt), stop=convert.ms_to_units(stop, dt),
+                    step=convert.ms_to_units(step, dt), dt=dt, dx=dx, params=params,
+                    diffusivity=diffusivity, stimuli=stimuli, filename=filepath,
+                    reshape=reshape, use_memory=use_memory, plot_while=plot_while)
 
 
 def sequence(start, stop, step, dt, dx, params, diffusivity, stimuli, filename,

@@ -103,3 +103,28 @@ def test_sequence_file_layout(gpu, tmp_path, monkeypatch, use_memory):
         assert np.array_equal(flat["field"][i], cio.imresize(s.field, (24, 28)).cpu().numpy())
         assert float(flat["start"][i]) == float(s.protocol.start)
     assert float(flat["params/tau_d"]) == pytest.approx(float(sim.params.tau_d))
+
+
+def test_random_sequence_end_to_end(gpu, tmp_path, monkeypatch):
+    """deepx/generate.py:86-132 -> sequence: seeded random stimuli and D-map,
+    frames in the reference's file layout, the same frames for the same seed."""
+    from cardiax_amd.params import PARAMSET_3
+
+    monkeypatch.setattr(cio, "_h5py", lambda: fake_h5py)
+    path = str(tmp_path / "rand.hdf5")
+    ret = generate.random_sequence(7, PARAMSET_3, path, shape=(64, 64), n_stimuli=1, start=0,
+                                   stop=5, step=1, reshape=(32, 32), use_memory=True,
+                                   plot_while=False)
+    # stop = 5 ms = 500 steps, a checkpoint every 100 steps: 5 checkpoints, 4 frames
+    assert ret.shape == (4, 3, 32, 32) and np.isfinite(ret).all()
+    flat = fake_h5py.File(path, "r").flat()
+    assert flat["states"].shape == (5, 3, 32, 32)
+    assert np.array_equal(flat["states"][:4], ret)
+    assert float(flat["start"][0]) == 1.0  # the first stimulus starts at step 1
+    D = flat["diffusivity"]
+    assert D.min() >= 1e-4 * (1 - 1e-6) and D.max() <= 1e-3 * (1 + 1e-6)
+    again = generate.random_sequence(7, PARAMSET_3, None, shape=(64, 64), n_stimuli=1,
+                                     start=0, stop=5, step=1, reshape=(32, 32),
+                                     plot_while=False)
+    assert np.array_equal(again, ret)
+    assert ret[-1, 2].max() > 0.1  # the stimulus at step 1 has excited the tissue
