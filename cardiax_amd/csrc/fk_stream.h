@@ -171,6 +171,16 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
       st((float*)((char*)a + band), off + (uint32_t)k * rb, x);
   };
 
+  // Precomputed coefficients of output row k (PC): float4 rows of pitch 16W
+  // bytes, addressed like the fields (clamped scalar row base on edge bands;
+  // one band base and a shared 32-bit row offset inside).
+  auto ldc = [&](int k) {
+    if constexpr (XG || FIELD_MAJOR)
+      return ld4(cf + (size_t)clampi(rs + k, 0, H - 1) * (size_t)W, off * 4u);
+    else
+      return ld4(cf + (size_t)rs * (size_t)W, off * 4u + (uint32_t)k * (rb * 4u));
+  };
+
   float uu[R + 8];           // u rows rs-4 .. rs+R+3
   float dd[PC ? 1 : R + 4];  // D rows rs-2 .. rs+R+1
   float4 cc[PC ? R : 1];     // precomputed coefficients of rows rs .. rs+R-1
@@ -188,32 +198,34 @@ __device__ __forceinline__ void stream_tile(const Coef& c, const float* __restri
       vv[k] = ldk(v, k + 4);
       ww[k] = ldk(w, k + 4);
     }
+    if constexpr (PC) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) cc[k] = ldc(k);
+    }
   } else {
     // In the order the output rows need them: first everything output row 0
-    // reads (u rows 0..8 of the window with D rows 2..6 alongside, then v and
-    // w of row 4), then the four rows each further output row adds. Loads
-    // return in order, so an output row's arithmetic starts once its own rows
-    // have arrived while the band's later rows are still in flight.
+    // reads (u rows 0..8 of the window with D rows 2..6 alongside, or its
+    // coefficients, then v and w of row 4), then the rows each further output
+    // row adds. Loads return in order, so an output row's arithmetic starts
+    // once its own rows have arrived while the band's later rows are still in
+    // flight.
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       uu[k] = ldk(u, k);
       if constexpr (!PC)
         if (k >= 2 && k < 7) dd[k - 2] = ldk(D, k);
     }
+    if constexpr (PC) cc[0] = ldc(0);
     vv[0] = ldk(v, 4);
     ww[0] = ldk(w, 4);
 #pragma unroll
     for (int j = 1; j < R; ++j) {
       uu[j + 8] = ldk(u, j + 8);
       if constexpr (!PC) dd[j + 4] = ldk(D, j + 6);
+      if constexpr (PC) cc[j] = ldc(j);
       vv[j] = ldk(v, j + 4);
       ww[j] = ldk(w, j + 4);
     }
-  }
-  if constexpr (PC) {
-#pragma unroll
-    for (int k = 0; k < R; ++k)
-      cc[k] = ld4(cf + (size_t)clampi(rs + k, 0, H - 1) * (size_t)W, off * 4u);
   }
   float h0[HEUN == 2 ? 3 * R : 1], hk[HEUN == 2 ? 3 * R : 1];  // stage 2: x0, k1
   if constexpr (HEUN == 2) {
