@@ -13,7 +13,7 @@ steps has the checkerboard structure that the reference's explicit scheme
 with its non-conservative D_x u_x term does not damp. The interpolation is
 two small matrix products (deterministic, unlike scattered atomics).
 
-The reference has no such driver (its deepx package learns the inverse map
+The reference has no such solver loop (its deepx package learns the inverse map
 with a network instead). The instances run interleaved on several HIP streams
 so that small grids, which leave most of the GPU idle, overlap; no step of
 the loop synchronises the host with the device.

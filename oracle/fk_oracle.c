@@ -2,7 +2,7 @@
  * C restatement of the cardiax forward-Euler path (oracle, CPU only).
  *
  * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py, always as the checker or the CPU reference
+ * cpu_baseline leg of bench.py, always to check results or as the CPU reference
  * timing, never by the product path.
  *
  * It keeps the reference's operation graph (cardiax/solve.py:26-70): edge-pad

@@ -1,8 +1,8 @@
 """CPU restatement of the cardiax forward-Euler path: the parity oracle.
 
 TEST INFRASTRUCTURE ONLY. Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
-checker. The product path (``cardiax_amd``) never imports it and has no CPU
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only to check
+results. The product path (``cardiax_amd``) never imports it and has no CPU
 fallback.
 
 Every function below restates one reference function in NumPy. The default

@@ -1,8 +1,8 @@
 """CPU restatement of the dataset generator's output: the resize applied to
 every checkpoint and the file layout it is written in.
 
-TEST INFRASTRUCTURE ONLY. Only ``tests/`` may import this module, as the
-checker; the product path (``cardiax_amd``) never does.
+TEST INFRASTRUCTURE ONLY. Only ``tests/`` may import this module, to check
+results; the product path (``cardiax_amd``) never does.
 
 Reference behaviour followed (file:line under the reference tree):
   * ``imresize``   cardiax/io.py:118-124: ``jax.image.resize(a, a.shape[:-2] +

@@ -1,8 +1,8 @@
 """CPU restatement of the rotation that cardiax's ``triangular`` stimulus
 applies to a linear stripe.
 
-TEST INFRASTRUCTURE ONLY. Only ``tests/`` may import this module, as the
-checker; the product path (``cardiax_amd``) never does.
+TEST INFRASTRUCTURE ONLY. Only ``tests/`` may import this module, to check
+results; the product path (``cardiax_amd``) never does.
 
 Reference behaviour followed (file:line under the reference tree):
   * ``rotate``  cardiax/stimulus.py:137-139: ``scipy.ndimage.rotate(field,
