@@ -42,7 +42,9 @@ def test_forward_autotune_records_and_is_used(gpu):
                          "streamc4", "streamc8", "streamc16", "pair2", "pair4", "tile16",
                          "tile32", "tb64x2", "tb64x4", "tb32x2", "tb32x4"}
         assert all(t > 0 for t in res["times_ms"].values())
-        assert res["best"] == min(res["times_ms"], key=res["times_ms"].get)
+        # the library takes the strict minimum of the unrounded times; the
+        # reported times are rounded to 1e-4 ms, so near-ties may reorder
+        assert res["times_ms"][res["best"]] <= min(res["times_ms"].values()) + 1e-4
         assert autotune.lookup(shape) == res["best"]
         assert autotune.lookup((97, 130)) == "auto"
         # an automatic solve on the tuned shape stays within the parity bar
