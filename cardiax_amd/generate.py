@@ -19,9 +19,10 @@ The reference runs ``_forward_euler`` per checkpoint interval, pulls each
 state to the host and resizes it there with ``jax.image.resize`` before
 writing it to HDF5. Here the state stays in HBM for the whole run, and each
 checkpoint is resized on the GPU (``fk_resize``) into one of two device frame
-buffers. The frame is then copied to pinned host memory on a copy stream,
-while the next interval's kernels are already queued. The host writes frame
-i - 1 (h5py) while the GPU works on interval i.
+buffers. The frame is then copied to one of two pinned host slots on a copy
+stream, while the next interval's kernels are already queued. The host moves
+frame i - 1 out of its slot (into the returned array, or into the file) while
+the GPU works on interval i.
 """
 from __future__ import annotations
 
@@ -169,7 +170,19 @@ def sequence(start, stop, step, dt, dx, params, diffusivity, stimuli, filename,
     n = max(len(checkpoints) - 1, 0)
     h, w = out_shape
     keep = hdf5 is None or use_memory  # every frame stays on the host
-    host = torch.empty((n if keep else 2, 3, h, w), dtype=torch.float32, pin_memory=True)
+    # Two pinned slots carry the frames off the device; a finished frame is
+    # copied into `frames` (pageable) or written to the file while the GPU
+    # runs the next interval.
+    host = torch.empty((2, 3, h, w), dtype=torch.float32, pin_memory=True)
+    frames = np.empty((n, 3, h, w), dtype=np.float32) if keep else None
+
+    def retire(k):  # frame k has reached its pinned slot
+        copied[k % 2].synchronize()
+        if keep:
+            frames[k] = host[k % 2].numpy()
+        else:
+            io.add_state(hdf5["states"], host[k % 2].numpy(), k)
+
     dev_frames = [torch.empty((3, h, w), dtype=torch.float32, device=dev) for _ in range(2)]
     work = torch.empty(max(1, io_workspace(shape, out_shape)), dtype=torch.uint8, device=dev)
     compute = torch.cuda.current_stream()
@@ -191,25 +204,22 @@ def sequence(start, stop, step, dt, dx, params, diffusivity, stimuli, filename,
         ready = torch.cuda.Event()
         ready.record(compute)
         copy.wait_event(ready)
-        slot = i if keep else i % 2  # a slot is reused after its frame was written
-        with torch.cuda.stream(copy):
-            host[slot].copy_(buf, non_blocking=True)
+        with torch.cuda.stream(copy):  # slot i % 2 was retired at the previous interval
+            host[i % 2].copy_(buf, non_blocking=True)
         done = torch.cuda.Event()
         done.record(copy)
         copied[i % 2] = done
-        if not keep and i > 0:  # write the previous frame while the GPU runs this one
-            copied[(i - 1) % 2].synchronize()
-            io.add_state(hdf5["states"], host[(i - 1) % 2].numpy(), i - 1)
+        if i > 0:  # retire the previous frame while the GPU runs this interval
+            retire(i - 1)
+    if n > 0:
+        retire(n - 1)
     torch.cuda.synchronize()
-    if hdf5 is not None:
-        if use_memory:
-            io.add_states(hdf5["states"], host.numpy(), 0, n)
-        elif n > 0:
-            io.add_state(hdf5["states"], host[(n - 1) % 2].numpy(), n - 1)
+    if hdf5 is not None and use_memory:
+        io.add_states(hdf5["states"], frames, 0, n)
     print()
     if hdf5 is not None:
         hdf5.close()
-    return host.numpy() if keep else None
+    return frames
 
 
 def io_workspace(shape, out_shape) -> int:
