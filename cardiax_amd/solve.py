@@ -211,6 +211,14 @@ def _forward_heun(state, t, t_end, params, diffusivity, stimuli, dt, dx) -> Stat
 _forward_dormandprince = _unsupported_integrator("DORMANDPRINCE")
 
 
+def step_rk(state, t, params, diffusivity, stimuli, dt, dx):
+    """cardiax/solve.py:86-87 integrates with jax.experimental.ode.odeint
+    (adaptive Dormand-Prince), which is outside this build's forward-Euler
+    path; raises NotImplementedError, as TimeIntegrator.DORMANDPRINCE does."""
+    raise NotImplementedError("step_rk (jax.experimental.ode.odeint) is outside this build's "
+                              "forward-Euler path")
+
+
 class TimeIntegrator:
     """cardiax/solve.py:127-130. As in the reference, each attribute *is* the
     integrator callable, so ``forward(..., integrator=TimeIntegrator.EULER)``

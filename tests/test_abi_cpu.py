@@ -169,12 +169,14 @@ def test_time_integrator_members_are_the_callables():
         solve.step_heun(st, 0, PARAMSET_3, np.full((8, 8), 1e-3), [], 0.01, 0.01)
     with pytest.raises(NotImplementedError):
         solve.TimeIntegrator.DORMANDPRINCE(None, [0, 1], None, None, None, 0.01, 0.01)
+    with pytest.raises(NotImplementedError):
+        solve.step_rk(st, 0, PARAMSET_3, np.full((8, 8), 1e-3), [], 0.01, 0.01)
 
 
 def test_public_surface_matches_reference_names():
     for name in ("State", "init", "step", "step_euler", "step_heun", "_forward_euler",
                  "_forward_heun", "TimeIntegrator",
-                 "forward", "forward_dimensional", "gradient", "stimulate"):
+                 "forward", "forward_dimensional", "gradient", "stimulate", "step_rk"):
         assert hasattr(solve, name), name
     assert solve.State._fields == ("v", "w", "u")
     assert stimulus.Protocol._fields == ("start", "duration", "period")
