@@ -163,10 +163,23 @@ def sequence(start, stop, step, dt, dx, params, diffusivity, stimuli, filename,
     hdf5 = None
     if filename is not None:
         hdf5 = io.init(filename, out_shape, n_iter=len(checkpoints), n_stimuli=len(stim))
-        io.add_params(hdf5, params, D, dt, dx, shape=out_shape)
-        io.add_stimuli(hdf5, stim, shape=out_shape)
-        io.add_diffusivity(hdf5, D, shape=out_shape)
+    try:
+        if hdf5 is not None:
+            io.add_params(hdf5, params, D, dt, dx, shape=out_shape)
+            io.add_stimuli(hdf5, stim, shape=out_shape)
+            io.add_diffusivity(hdf5, D, shape=out_shape)
+        return _sequence_frames(state_args=(params, D, stim, dt, dx), checkpoints=checkpoints,
+                                shape=shape, out_shape=out_shape, hdf5=hdf5,
+                                use_memory=use_memory, dev=dev)
+    finally:
+        if hdf5 is not None:
+            hdf5.close()
 
+
+def _sequence_frames(state_args, checkpoints, shape, out_shape, hdf5, use_memory, dev):
+    """The checkpoint loop of ``sequence``: frames to ``hdf5`` (or kept and
+    returned), the file's closing left to the caller."""
+    params, D, stim, dt, dx = state_args
     n = max(len(checkpoints) - 1, 0)
     h, w = out_shape
     keep = hdf5 is None or use_memory  # every frame stays on the host
@@ -217,8 +230,6 @@ def sequence(start, stop, step, dt, dx, params, diffusivity, stimuli, filename,
     if hdf5 is not None and use_memory:
         io.add_states(hdf5["states"], frames, 0, n)
     print()
-    if hdf5 is not None:
-        hdf5.close()
     return frames
 
 

@@ -198,6 +198,8 @@ def imresize(a, size, method="bilinear", out=None, work=None):
                          % (lead + (h, w),))
     L = _lib.lib()
     nbytes = int(L.fk_resize_workspace_bytes(n, H, W, h, w))
+    if work is not None and (work.device != x.device or not work.is_contiguous()):
+        raise ValueError("imresize: work must be a contiguous tensor on %s" % (x.device,))
     if work is None or work.numel() * work.element_size() < nbytes:
         work = torch.empty(max(1, nbytes), dtype=torch.uint8, device=dev)
     _lib.check(L.fk_resize(x.data_ptr(), n, H, W, out.data_ptr(), h, w, work.data_ptr(),

@@ -50,6 +50,12 @@ def test_resize_leading_axes_and_reuse(gpu):
         cio.imresize(a, (16, 12), out=torch.empty((2, 3, 16, 11), device="cuda"))
     with pytest.raises(ValueError):
         cio.imresize(a[0, 0, 0], (2, 2))
+    # a caller's workspace gives the same bits; one in host memory is refused before any launch
+    work = torch.empty(generate.io_workspace((40, 30), (16, 12)) * 6, dtype=torch.uint8,
+                       device="cuda")
+    assert torch.equal(cio.imresize(a, (16, 12), work=work), out)
+    with pytest.raises(ValueError):
+        cio.imresize(a, (16, 12), work=torch.empty(work.numel(), dtype=torch.uint8))
 
 
 def _small_run(seed=2, shape=(64, 64)):
