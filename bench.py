@@ -271,6 +271,48 @@ def bench_heun(seed, size=1024, n_steps=2000):
             "u_finite": bool(torch.isfinite(out.u).all().item())}
 
 
+def bench_slab(seed, n_steps, size=1024, n_parts=8, depths=(1, 4, 8), tune=True):
+    """One GPU's share of the size x size tissue split into n_parts row slabs:
+    size / n_parts owned rows plus 4k ghost rows on each side, the depth a
+    halo exchange every k steps needs (the stencil reaches 4 rows per step).
+    The slab's rows come from the middle of the single-tissue workload. GCUPS
+    count owned cells only; the ghost rows are the recomputation that k steps
+    between exchanges cost. No exchange is made: there is one GPU."""
+    import numpy as np
+    import torch
+
+    from cardiax_amd import autotune, solve, workloads
+
+    sim = workloads.c2_inputs(seed, shape=(size, size), steps=n_steps)
+    dev = torch.device("cuda", 0)
+    own = size // n_parts
+    r_own = (size - own) // 2
+    res = {"workload": "%d of %d row slabs of a %dx%d grid (%d owned rows), %d Euler steps"
+                       % (1, n_parts, size, size, own, n_steps), "slabs": []}
+    for k in depths:
+        r0, r1 = r_own - 4 * k, r_own + own + 4 * k
+        shape = (r1 - r0, size)
+        D = torch.from_numpy(np.ascontiguousarray(sim.D[r0:r1])).to(dev)
+        stims = [(s.protocol, torch.from_numpy(np.ascontiguousarray(s.field[r0:r1])).to(dev))
+                 for s in sim.stimuli]
+        variant = autotune.tune(shape, n_steps=300)["best"] if tune else autotune.resolved(shape)
+        st0 = solve.init(shape)
+        solve._forward_euler(st0, 0, 200, sim.params, D, stims, sim.dt, sim.dx)  # warm-up
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        solve._forward_euler(st0, 0, n_steps, sim.params, D, stims, sim.dt, sim.dx)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / n_steps
+        res["slabs"].append({"steps_per_exchange": k, "rows": shape[0], "variant": variant,
+                             "us_per_step": round(us, 3),
+                             "owned_gcups": round(own * size / us / 1e3, 3),
+                             "exchange_bytes_per_neighbour": 4 * k * size * 4})
+    return res
+
+
 def bench_sequence(seed, n_steps, size=1200, every=500, reshape=256):
     """deepx/generate.py:135-210 at the reference's production shape (1200^2
     grid, a checkpoint every 500 steps, frames resized to 256^2,
@@ -350,6 +392,8 @@ def main():
                     help="skip the autotuner (the size-based default kernel runs)")
     ap.add_argument("--sequence-steps", type=int, default=20_000,
                     help="Euler steps of the dataset-generator leg (0 skips it)")
+    ap.add_argument("--slab-steps", type=int, default=4000,
+                    help="Euler steps of the one-slab-of-eight leg (0 skips it)")
     args = ap.parse_args()
     # stdout carries the one JSON line; the library's progress prints go to stderr
     json_out = sys.stdout
@@ -461,6 +505,11 @@ def main():
     log("heun ...")
     extra["heun"] = bench_heun(args.seed)
     log("heun: %s" % json.dumps(extra["heun"]))
+
+    if args.slab_steps > 0:
+        log("row slab ...")
+        extra["slab"] = bench_slab(args.seed, args.slab_steps, size=H, tune=not args.no_tune)
+        log("slab: %s" % json.dumps(extra["slab"]))
 
     if args.sequence_steps > 0:
         log("dataset generator ...")
