@@ -119,9 +119,10 @@ int32_t fk_affine_transform(const float* src, int32_t H, int32_t W, const double
   for (int k = 0; k < 4; ++k)
     if (!std::isfinite(matrix[k]) || (k < 2 && !std::isfinite(offset[k])))
       return set_error(FK_ERR_ARG, "fk_affine_transform: non-finite map");
-  if (src == dst) return set_error(FK_ERR_ARG, "fk_affine_transform: dst must differ from src");
-  const AffineArgs a{matrix[0], matrix[1], matrix[2], matrix[3], offset[0], offset[1]};
   const int64_t n = (int64_t)H * W;
+  if (dst < src + n && src < dst + n)
+    return set_error(FK_ERR_ARG, "fk_affine_transform: dst overlaps src");
+  const AffineArgs a{matrix[0], matrix[1], matrix[2], matrix[3], offset[0], offset[1]};
   k_affine_spline3<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(src, H, W, a,
                                                                                  dst);
   HIP_CHECK(hipGetLastError());

@@ -355,7 +355,7 @@ int32_t fk_stimulus_box(float* field, int32_t H, int32_t W, int32_t r0, int32_t 
  * host memory read before the call returns); taps outside the grid take the
  * nearest edge value. The caller builds the map as scipy's rotate does:
  * matrix = [[cos a, sin a], [-sin a, cos a]] (scipy.special.cosdg/sindg) and
- * offset = c - matrix @ c, c = ((H-1)/2, (W-1)/2). dst must differ from src.
+ * offset = c - matrix @ c, c = ((H-1)/2, (W-1)/2). dst must not overlap src.
  * Stream-ordered. */
 int32_t fk_affine_transform(const float* src, int32_t H, int32_t W, const double* matrix,
                             const double* offset, float* dst, void* stream);

@@ -135,6 +135,31 @@ def test_check_finite_arguments():
         _lib.check(_lib.FK_ERR_NONFINITE)
 
 
+def test_io_and_stimulus_argument_errors():
+    """Overlapping buffers, small workspaces and bad boxes are refused before
+    any launch (the pointers are never dereferenced)."""
+    L = _lib.lib()
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x1000 + 4 * 8 * 8 - 4)
+    far = ctypes.c_void_p(0x100000)
+    m = (ctypes.c_double * 4)(1.0, 0.0, 0.0, 1.0)
+    off = (ctypes.c_double * 2)(0.0, 0.0)
+    # 8 x 8 floats from 0x1000 overlap a dst starting one float before their end
+    assert L.fk_affine_transform(a, 8, 8, m, off, b, None) == _lib.FK_ERR_ARG
+    assert b"overlap" in L.fk_last_error()
+    assert L.fk_affine_transform(b, 8, 8, m, off, a, None) == _lib.FK_ERR_ARG
+    bad = (ctypes.c_double * 4)(float("nan"), 0.0, 0.0, 1.0)
+    assert L.fk_affine_transform(a, 8, 8, bad, off, far, None) == _lib.FK_ERR_ARG
+    need = L.fk_resize_workspace_bytes(1, 8, 8, 4, 4)
+    assert need == 4 * 8 * 8
+    assert L.fk_resize(a, 1, 8, 8, far, 4, 4, far, need - 1, None) == _lib.FK_ERR_ARG
+    assert b"workspace" in L.fk_last_error()
+    assert L.fk_resize(a, 1, 8, 8, b, 4, 4, far, need, None) == _lib.FK_ERR_ARG
+    assert b"overlap" in L.fk_last_error()
+    assert L.fk_resize(a, 0, 8, 8, b, 4, 4, None, 0, None) == _lib.FK_OK  # nothing to do
+    assert L.fk_stimulus_box(a, 8, 8, 0, 9, 0, 4, 1.0, None) == _lib.FK_ERR_ARG
+    assert L.fk_stimulus_box(a, 8, 8, -1, 4, 0, 4, 1.0, None) == _lib.FK_ERR_ARG
+
+
 def test_check_maps_codes_to_python_errors():
     _call_step(H=1)
     with pytest.raises(ValueError):
