@@ -244,8 +244,9 @@ def bench_inverse(seed, n=32, iters=50, n_steps=1000, frame_every=50):
 def bench_heun(seed, size=1024, n_steps=2000):
     """TimeIntegrator.HEUN (cardiax/solve.py:103-111) on the single-tissue
     workload: two streaming launches per step. Algorithmic bytes per Heun
-    cell-step: stage 1 reads v, w, u, D and writes x1 and k1 (40 B); stage 2
-    reads x1, D, x0 and k1 and writes the state (52 B)."""
+    cell-step: stage 1 reads v, w, u, D and writes k1 (28 B); stage 2 reads
+    x0, k1 and D, rebuilds x1 = x0 + dt k1 in registers and writes the state
+    (40 B): 68 B."""
     import torch
 
     from cardiax_amd import solve, workloads
@@ -267,7 +268,7 @@ def bench_heun(seed, size=1024, n_steps=2000):
     gcups = size * size * n_steps / sec / 1e9
     return {"workload": "%dx%d grid, %d Heun steps (two launches each)" % (size, size, n_steps),
             "gcups": round(gcups, 3), "us_per_step": round(sec / n_steps * 1e6, 3),
-            "hbm_frac": round(gcups * 92 / HBM_PEAK_GBS, 4),
+            "hbm_frac": round(gcups * 68 / HBM_PEAK_GBS, 4),
             "u_finite": bool(torch.isfinite(out.u).all().item())}
 
 

@@ -502,7 +502,7 @@ static hipError_t launch_heun(int stage, const Coef& c, const float* D, const fl
 }
 
 // Heun steps at times t .. t_end-1 (cardiax/solve.py:103-111) from `in` to
-// `out`; the workspace holds x1 and k1 (6 fields) and a ping-pong state.
+// `out`; the workspace holds k1 (3 fields) and a ping-pong state.
 static int32_t heun_loop(const fk_params* params, const float* D, const float* const in[3],
                          int32_t H, int32_t W, const fk_stimulus* stimuli, int32_t n_stimuli,
                          double t, int64_t steps, float dt, float dx, float* const out[3],
@@ -511,21 +511,20 @@ static int32_t heun_loop(const fk_params* params, const float* D, const float* c
   const Coef c = make_coef(*params);
   const float inv_dx2 = (float)(1.0 / ((double)dx * (double)dx));
   float* wk = (float*)work;
-  float* x1[3] = {wk, wk + n, wk + 2 * n};
-  float* k1[3] = {wk + 3 * n, wk + 4 * n, wk + 5 * n};
-  float* ping[3] = {wk + 6 * n, wk + 7 * n, wk + 8 * n};
+  float* k1[3] = {wk, wk + n, wk + 2 * n};
+  float* ping[3] = {wk + 3 * n, wk + 4 * n, wk + 5 * n};
   float* const* bufs[2] = {out, ping};
   int dst = (steps % 2 == 1) ? 0 : 1;  // the last step lands in `out`
   const float* src[3] = {in[0], in[1], in[2]};
   StimSchedule sched(stimuli, n_stimuli);
   for (int64_t k = 0; k < steps; ++k) {
     const StimArgs& sa = sched.at(t + (double)k);  // both stages at time t + k
-    HeunIO io1{k1[0], k1[1], k1[2], nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    HIP_CHECK(launch_heun(1, c, D, src, H, W, sa, dt, inv_dx2, x1, io1, s));
-    HeunIO io2{nullptr, nullptr, nullptr, src[0], src[1], src[2], k1[0], k1[1], k1[2]};
-    const float* x1c[3] = {x1[0], x1[1], x1[2]};
+    // stage 1 writes only k1 (its state outputs are not written)
+    HeunIO io1{k1[0], k1[1], k1[2], nullptr, nullptr, nullptr};
+    HIP_CHECK(launch_heun(1, c, D, src, H, W, sa, dt, inv_dx2, k1, io1, s));
+    HeunIO io2{nullptr, nullptr, nullptr, k1[0], k1[1], k1[2]};
     float* const* d = bufs[dst];
-    HIP_CHECK(launch_heun(2, c, D, x1c, H, W, sa, dt, inv_dx2, d, io2, s));
+    HIP_CHECK(launch_heun(2, c, D, src, H, W, sa, dt, inv_dx2, d, io2, s));
     src[0] = d[0];
     src[1] = d[1];
     src[2] = d[2];
@@ -827,7 +826,7 @@ int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const fl
 
 size_t fk_heun_workspace_bytes(int32_t H, int32_t W) {
   if (H <= 0 || W <= 0) return 0;
-  return 9 * (size_t)H * (size_t)W * sizeof(float);
+  return 6 * (size_t)H * (size_t)W * sizeof(float);
 }
 
 int32_t fk_step_heun(const fk_params* params, const float* D, const float* v, const float* w,

@@ -167,13 +167,14 @@ int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const fl
                                void* stream);
 
 /* Scratch bytes fk_step_heun and fk_forward_heun need for an H x W grid
- * (the stage-1 state and slope, and a ping-pong state: 9 fields). */
+ * (the stage-1 slope and a ping-pong state: 6 fields). */
 size_t fk_heun_workspace_bytes(int32_t H, int32_t W);
 
 /* cardiax/solve.py:73-85 -- one Heun step at time t: k1 = step(x, t),
  * x1 = x + dt k1, k2 = step(x1, t), out = x + (k1 + k2) dt / 2. Two fused
- * launches (the first writes x1 and k1 to the workspace). Outputs must not
- * alias inputs or the workspace. */
+ * launches: the first writes k1 to the workspace, the second rebuilds x1
+ * from x and k1 in registers. Outputs must not alias inputs or the
+ * workspace. */
 int32_t fk_step_heun(const fk_params* params, const float* D, const float* v, const float* w,
                      const float* u, int32_t H, int32_t W, const fk_stimulus* stimuli,
                      int32_t n_stimuli, double t, float dt, float dx, float* v_out,
