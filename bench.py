@@ -401,8 +401,8 @@ def main():
     sys.stdout = sys.stderr
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the solver has no multi-GPU path in this "
-              "build (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only: the solver has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -196,7 +196,8 @@ def _forward_euler_plain(state, t, t_end, params, diffusivity, stimuli, dt, dx) 
 def _unsupported_integrator(name):
     def f(*args, **kwargs):
         raise NotImplementedError(
-            "%s integration is outside this build's forward-Euler path" % name)
+            "%s integration is not implemented: this package provides the forward-Euler "
+            "and Heun integrators" % name)
     f.__name__ = "_forward_" + name.lower()
     return f
 
@@ -213,10 +214,10 @@ _forward_dormandprince = _unsupported_integrator("DORMANDPRINCE")
 
 def step_rk(state, t, params, diffusivity, stimuli, dt, dx):
     """cardiax/solve.py:86-87 integrates with jax.experimental.ode.odeint
-    (adaptive Dormand-Prince), which is outside this build's forward-Euler
-    path; raises NotImplementedError, as TimeIntegrator.DORMANDPRINCE does."""
-    raise NotImplementedError("step_rk (jax.experimental.ode.odeint) is outside this build's "
-                              "forward-Euler path")
+    (adaptive Dormand-Prince), which this package does not implement; raises
+    NotImplementedError, as TimeIntegrator.DORMANDPRINCE does."""
+    raise NotImplementedError("step_rk (jax.experimental.ode.odeint) is not implemented: this "
+                              "package provides the forward-Euler and Heun integrators")
 
 
 class TimeIntegrator:

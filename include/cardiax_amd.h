@@ -41,7 +41,7 @@ enum {
   FK_OK = 0,
   FK_ERR_ARG = -1,         /* invalid argument: shape, pointer, count, range */
   FK_ERR_HIP = -2,         /* a HIP runtime call failed */
-  FK_ERR_UNSUPPORTED = -3, /* valid request outside this build's limits */
+  FK_ERR_UNSUPPORTED = -3, /* valid request beyond the library's limits */
   FK_ERR_NONFINITE = -4    /* fk_check_finite found NaN or inf in device data */
 };
 
