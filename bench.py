@@ -266,9 +266,15 @@ def bench_heun(seed, size=1024, n_steps=2000):
     torch.cuda.synchronize()
     sec = e0.elapsed_time(e1) / 1e3
     gcups = size * size * n_steps / sec / 1e9
+    # HBM bytes per Heun step from the committed PMC summary of both stages (None if absent)
+    R = 2 if size * size <= 1536 * 1024 else 4
+    t1 = measured_traffic("k_stream_heun<%d, 1>" % R, size, size)
+    t2 = measured_traffic("k_stream_heun<%d, 2>" % R, size, size)
+    traffic = t1 + t2 if t1 is not None and t2 is not None else None
     return {"workload": "%dx%d grid, %d Heun steps (two launches each)" % (size, size, n_steps),
             "gcups": round(gcups, 3), "us_per_step": round(sec / n_steps * 1e6, 3),
             "hbm_frac": round(gcups * 68 / HBM_PEAK_GBS, 4),
+            "traffic_bytes_per_step": traffic,
             "u_finite": bool(torch.isfinite(out.u).all().item())}
 
 
