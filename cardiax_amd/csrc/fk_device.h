@@ -44,7 +44,8 @@ __device__ __forceinline__ float half_one_plus_tanh(const Coef& c, float u) {
 
 // Gates and ionic currents, cardiax/solve.py:35-42,57-58.
 // p = [u >= V_c], q = [u >= V_v]; outputs dv, dw and j_ion (before stimulus).
-// Both sides of every gate are evaluated and selected (no divergent branch).
+// The gates are written as selects; the compiler turns some of them into
+// branches, so a wave whose cells are all at rest skips the excited side.
 // Contraction is off in the shared step code: every kernel that inlines it
 // rounds identically, so batched and single-grid runs agree bit for bit.
 __device__ __forceinline__ void reaction(const Coef& c, float v, float w, float u, float& dv,
