@@ -23,6 +23,7 @@ FK_ERR_HIP = -2
 FK_ERR_UNSUPPORTED = -3
 FK_ERR_NONFINITE = -4
 FK_MAX_STIMULI = 32
+FK_NOALLOC_MAX_CELLS = 1 << 20
 
 # Forward kernel variants (fk_set_forward_variant).
 VARIANTS = {"auto": 0, "plain": 1, "tile16": 2, "tile32": 3, "stream8": 4, "stream16": 5,
@@ -80,6 +81,9 @@ _SIGNATURES = {
     "fk_forward_euler": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp, c_i32,
                                  c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d, c_d, c_f, c_f,
                                  c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "fk_forward_euler_noalloc": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp,
+                                         c_i32, c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d,
+                                         c_d, c_f, c_f, c_vp, c_vp, c_vp, c_vp]),
     "fk_forward_euler_plain": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp,
                                        c_i32, c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d,
                                        c_d, c_f, c_f, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),

@@ -331,6 +331,8 @@ static bool is_tb(int v) { return v >= kTb64x2 && v <= kTb32x4; }
 
 static std::atomic<int> g_variant{kAuto};
 
+int current_forward_variant() { return g_variant.load(); }
+
 // Autotuner results: grid shape -> fastest variant measured on this device
 // (fk_autotune_forward). Read on every automatic launch, written by the
 // tuner; guarded by a mutex.
@@ -533,12 +535,11 @@ static int32_t heun_loop(const fk_params* params, const float* D, const float* c
   return FK_OK;
 }
 
-static int32_t forward_loop(int variant, const fk_params* params, const float* D,
-                            const float* v_in, const float* w_in, const float* u_in, int32_t H,
-                            int32_t W, const fk_stimulus* stimuli, int32_t n_stimuli,
-                            double t, double t_end, float dt, float dx, float* v_out,
-                            float* w_out, float* u_out, void* work, size_t work_bytes,
-                            void* stream) {
+int32_t forward_loop(int variant, const fk_params* params, const float* D, const float* v_in,
+                     const float* w_in, const float* u_in, int32_t H, int32_t W,
+                     const fk_stimulus* stimuli, int32_t n_stimuli, double t, double t_end,
+                     float dt, float dx, float* v_out, float* w_out, float* u_out, void* work,
+                     size_t work_bytes, void* stream, bool state_only) {
   ErrorScope es;
   if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
       !check_ptrs({D, v_in, w_in, u_in, v_out, w_out, u_out}) || !check_time(t, t_end) ||
@@ -546,9 +547,10 @@ static int32_t forward_loop(int variant, const fk_params* params, const float* D
     return last_code();
   const size_t n = (size_t)H * W;
   const int64_t steps = loop_count(t, t_end);
-  if (steps >= 2 && (work == nullptr || work_bytes < fk_forward_workspace_bytes(H, W)))
-    return set_error(FK_ERR_ARG, "workspace too small: need %zu bytes, got %zu",
-                     fk_forward_workspace_bytes(H, W), work_bytes);
+  const size_t need = state_only ? 3 * n * sizeof(float) : fk_forward_workspace_bytes(H, W);
+  if (steps >= 2 && (work == nullptr || work_bytes < need))
+    return set_error(FK_ERR_ARG, "workspace too small: need %zu bytes, got %zu", need,
+                     work_bytes);
   hipStream_t s = (hipStream_t)stream;
   if (steps == 0) {
     for (auto pr : {std::make_pair(v_in, v_out), std::make_pair(w_in, w_out),
@@ -562,7 +564,8 @@ static int32_t forward_loop(int variant, const fk_params* params, const float* D
   float* wk = (float*)work;
   float* bufs[2][3] = {{v_out, w_out, u_out}, {wk, wk + n, wk + 2 * n}};
   int v = resolve_for(variant, H, W);
-  if (is_pc(v) && steps < 2) v = pc_base(v);  // nothing to amortise
+  // nothing to amortise, or no room for the coefficients
+  if (is_pc(v) && (steps < 2 || state_only)) v = pc_base(v);
   float4* cf = nullptr;
   if (is_pc(v)) {
     cf = (float4*)((char*)work + coef_offset(H, W));
@@ -701,7 +704,7 @@ int32_t fk_forward_euler(const fk_params* params, const float* D, const float* v
                          double t_end, float dt, float dx, float* v_out, float* w_out,
                          float* u_out, void* work, size_t work_bytes, void* stream) {
   return forward_loop(g_variant.load(), params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli, t,
-                      t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
+                      t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream, false);
 }
 
 int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
@@ -737,11 +740,11 @@ int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
     if (e != hipSuccess) break;
     // warm-up launch, then the timed run
     int32_t rc = forward_loop(v, &p, D, x, x + n, x + 2 * n, H, W, nullptr, 0, 0.0, 8.0, 0.01f,
-                              0.01f, y, y + n, y + 2 * n, wk, wb, s);
+                              0.01f, y, y + n, y + 2 * n, wk, wb, s, false);
     if (rc == FK_OK) e = hipEventRecord(e0, s);
     if (rc == FK_OK && e == hipSuccess)
       rc = forward_loop(v, &p, D, x, x + n, x + 2 * n, H, W, nullptr, 0, 0.0, (double)n_steps,
-                        0.01f, 0.01f, y, y + n, y + 2 * n, wk, wb, s);
+                        0.01f, 0.01f, y, y + n, y + 2 * n, wk, wb, s, false);
     if (rc == FK_OK && e == hipSuccess) e = hipEventRecord(e1, s);
     if (rc == FK_OK && e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.0f;
@@ -821,7 +824,7 @@ int32_t fk_forward_euler_plain(const fk_params* params, const float* D, const fl
                                float* w_out, float* u_out, void* work, size_t work_bytes,
                                void* stream) {
   return forward_loop(kPlain, params, D, v_in, w_in, u_in, H, W, stimuli, n_stimuli,
-                      t, t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream);
+                      t, t_end, dt, dx, v_out, w_out, u_out, work, work_bytes, stream, false);
 }
 
 size_t fk_heun_workspace_bytes(int32_t H, int32_t W) {

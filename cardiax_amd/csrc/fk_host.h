@@ -186,6 +186,20 @@ class StimSchedule {
   StimArgs args_;
 };
 
+// fk_forward.hip: the Euler loop behind fk_forward_euler (cardiax/solve.py:92-100)
+// with the kernel `variant` (0 = automatic). The workspace holds a ping-pong
+// state and, unless `state_only`, the precomputed diffusion coefficients;
+// with `state_only` a precomputed-coefficient variant runs its streaming
+// counterpart and the workspace needs only 3 * H * W floats.
+int32_t forward_loop(int variant, const fk_params* params, const float* D, const float* v_in,
+                     const float* w_in, const float* u_in, int32_t H, int32_t W,
+                     const fk_stimulus* stimuli, int32_t n_stimuli, double t, double t_end,
+                     float dt, float dx, float* v_out, float* w_out, float* u_out, void* work,
+                     size_t work_bytes, void* stream, bool state_only);
+
+// The process-wide fk_set_forward_variant setting.
+int current_forward_variant();
+
 inline Coef make_coef(const fk_params& p) {
   const double log2e = 1.4426950408889634;
   Coef c;

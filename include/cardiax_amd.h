@@ -156,6 +156,27 @@ int32_t fk_forward_euler(const fk_params* params, const float* D, const float* v
                          double t_end, float dt, float dx, float* v_out, float* w_out,
                          float* u_out, void* work, size_t work_bytes, void* stream);
 
+/* Most cells fk_forward_euler_noalloc accepts: 2^20 (a 1024 x 1024 grid). */
+#define FK_NOALLOC_MAX_CELLS (1 << 20)
+
+/* No reference counterpart: fk_forward_euler for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same loop, arguments and results, without the
+ * workspace: the ping-pong state lives in a fixed device array of
+ * 3 * FK_NOALLOC_MAX_CELLS floats (12 MiB) that each GPU the library runs on
+ * holds until the process ends. The call allocates nothing. Grids above
+ * FK_NOALLOC_MAX_CELLS cells give FK_ERR_ARG. A variant with precomputed
+ * diffusion coefficients (11, 12, 14, 16) runs its streaming counterpart
+ * (8, 16, 4, 2 rows), which has no room to keep them. `stream` (NULL = the
+ * null stream) must belong to the current device. Every call on one GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (other
+ * streams, other host threads) race and give wrong results. */
+int32_t fk_forward_euler_noalloc(const fk_params* params, const float* D, const float* v_in,
+                                 const float* w_in, const float* u_in, int32_t H, int32_t W,
+                                 const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                                 double t_end, float dt, float dx, float* v_out,
+                                 float* w_out, float* u_out, void* stream);
+
 /* Same loop with the plain one-thread-per-cell kernel (global loads, no LDS,
  * one launch per step): the unoptimised fp32 baseline the tuned kernels are
  * measured against. Same contract as fk_forward_euler. */
