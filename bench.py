@@ -44,10 +44,15 @@ def measured_traffic(kernel, H, W):
     """HBM bytes per launch of `kernel` at H x W from the committed rocprofv3
     --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known
     size); None when no summary covers this kernel and size."""
+    return measured_traffic_key("%s %dx%d" % (kernel, H, W))
+
+
+def measured_traffic_key(what):
+    """The `traffic_bytes_per_launch[what] = N` line of a summary in profiles/."""
     import glob
     import re
 
-    key = "traffic_bytes_per_launch[%s %dx%d]" % (kernel, H, W)
+    key = "traffic_bytes_per_launch[%s]" % what
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(path) as f:
             for line in f:
@@ -58,15 +63,26 @@ def measured_traffic(kernel, H, W):
     return None
 
 
-def cpu_cores():
+def cpu_affinity():
+    """Cores this process may run on."""
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return os.cpu_count() or 1
 
 
-def cpu_baseline(sim, target_s=12.0):
+def cpu_cores():
+    """Threads of the CPU baseline: every core of the affinity mask, bounded
+    by OMP_NUM_THREADS where the host sets it (the CPU share of one GPU on a
+    shared node)."""
+    n = cpu_affinity()
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(sim, target_s=15.0):
     """fp32 C restatement of the reference op graph (oracle/fk_oracle.c) on
     the host cores, over a bounded sample of the same workload."""
     import numpy as np
@@ -82,7 +98,7 @@ def cpu_baseline(sim, target_s=12.0):
     st = c_oracle.forward_euler(st, 0, n0, sim.params, sim.D, stim, sim.dt, sim.dx,
                                 np.float32, cores)
     dt0 = time.perf_counter() - t0
-    n1 = int(max(10, min(5000, n0 * target_s / max(dt0, 1e-6))))
+    n1 = int(max(10, min(50000, n0 * target_s / max(dt0, 1e-6))))
     t0 = time.perf_counter()
     c_oracle.forward_euler(st, n0, n0 + n1, sim.params, sim.D, stim, sim.dt, sim.dx,
                            np.float32, cores)
@@ -91,19 +107,37 @@ def cpu_baseline(sim, target_s=12.0):
             "sample": "%dx%d grid, Euler steps %d..%d of the same inputs, fp32 C "
                       "restatement of the reference op graph, %d OpenMP threads, %.1f s"
                       % (H, W, n0, n0 + n1, cores, el),
-            "cpu_model": cpu_model()}
+            "cpu_model": cpu_model(), "cores_in_affinity": cpu_affinity(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def cpu_model():
-    """Model name of the host CPU (/proc/cpuinfo), as lscpu reports it."""
+    """Model name of the host CPU as lscpu reports it (/proc/cpuinfo when lscpu
+    is absent), with the socket, core and thread counts."""
+    import subprocess
+
+    info = {}
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
         pass
-    return None
+    name = info.get("Model name")
+    if not name:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        name = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    extra = ", ".join("%s %s" % (k.lower(), info[k]) for k in
+                      ("Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                       "CPU max MHz") if info.get(k))
+    return "%s (%s)" % (name, extra) if name and extra else name
 
 
 def bench_batched(n_sims, n_steps, seed, per_sim_steps=64):
@@ -155,7 +189,21 @@ def bench_batched(n_sims, n_steps, seed, per_sim_steps=64):
             "speedup_vs_per_sim": round(batched / per_sim, 2), "autotune": tuned,
             "u_finite": n_bad == 0,
             "n_nonfinite_sims": n_bad, "u_absmax": round(umax, 4),
-            "hbm_frac": round(batched * BYTES_PER_CELL_UPDATE / HBM_PEAK_GBS, 4)}
+            "roofline": leg_roofline(BYTES_PER_CELL_UPDATE * cells, el / n_steps,
+                                     "k_batch_step<%d>" % tuned["rows"],
+                                     "%d simulations, %.1f M cells" % (n_sims, cells / 1e6))}
+
+
+def leg_roofline(bytes_per_launch, seconds_per_launch, kernel, key):
+    """HBM roofline of a leg's dominant kernel: algorithmic bytes per launch
+    over the measured time per launch, with the traffic a committed
+    rocprofv3 --pmc summary measured for it (None when there is none)."""
+    achieved = bytes_per_launch / seconds_per_launch / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic_key("%s %s" % (kernel, key)), "kernel": kernel,
+            "per_launch_us": round(seconds_per_launch * 1e6, 3),
+            "bytes_per_launch": int(bytes_per_launch)}
 
 
 def bench_adjoint(seed, size=256, n_steps=2000, frame_every=50, reps=2):
@@ -203,6 +251,8 @@ def bench_adjoint(seed, size=256, n_steps=2000, frame_every=50, reps=2):
     torch.cuda.synchronize()
     tb = (time.perf_counter() - t0) / reps
     cells = size * size * n_steps
+    per_step = adjoint_step_seconds(st, prm, Dstar, stims, shape, t_mid=n_steps // 2)
+    fwd_adj = ADJ_FWD_BWD_BYTES * cells / tb / 1e9
     return {"workload": "%dx%d, %d Euler steps, u frames every %d, MSE vs a hidden scar D-map, "
                         "grad wrt D and tau_d/tau_r/tau_si, sqrt(N) checkpoints"
                         % (size, size, n_steps, frame_every),
@@ -210,7 +260,70 @@ def bench_adjoint(seed, size=256, n_steps=2000, frame_every=50, reps=2):
             "cost_ratio": round(tb / tf, 2),
             "forward_gcups": round(cells / tf / 1e9, 3),
             "fwd_adjoint_gcups": round(cells / tb / 1e9, 3),
+            "fwd_adjoint_hbm": {"bytes_per_cell_step": ADJ_FWD_BWD_BYTES,
+                                "achieved": round(fwd_adj, 1),
+                                "frac": round(fwd_adj / HBM_PEAK_GBS, 4)},
+            "roofline": leg_roofline(ADJ_BWD_BYTES * size * size, per_step,
+                                     "k_adj_main<true>", "%dx%d" % shape),
             "grad_finite": bool(torch.isfinite(D.grad).all().item())}
+
+
+# Algorithmic bytes of the adjoint per cell-step: the reverse step reads the
+# cotangent (12), the state (12) and D (4), writes the cotangent (12) and
+# read-modify-writes dL/dD (8); forward + backward adds the forward step (28)
+# and its recompute from the checkpoints (28).
+ADJ_BWD_BYTES = 48
+ADJ_FWD_BWD_BYTES = 104
+
+
+def adjoint_step_seconds(st, prm, D, stims, shape, t_mid, n_short=10, n_long=40):
+    """Time of one reverse step (k_adj_main<true>) on the stream it runs on:
+    fk_adjoint_segment over n_long and n_short steps of a trajectory from the
+    middle of the rollout, HIP events around each; the difference cancels the
+    segment's fixed launches."""
+    import ctypes
+
+    import torch
+
+    from cardiax_amd import _lib, solve
+
+    dev = torch.device("cuda", 0)
+    H, W = shape
+    L = _lib.lib()
+    x = solve._forward_euler(st, 0, t_mid, prm, D, stims, 0.01, 0.01)
+    stim = solve._Stimuli(stims, dev, shape)
+    p = _lib.make_params(prm)
+    Dd = torch.as_tensor(D, device=dev)
+    s = torch.cuda.current_stream()
+    states = torch.empty((n_long, 3, H, W), device=dev)
+    for f in range(3):
+        states[0, f].copy_(x[f])
+    _lib.check(L.fk_forward_trajectory(ctypes.byref(p), Dd.data_ptr(), x.v.data_ptr(),
+                                       x.w.data_ptr(), x.u.data_ptr(), H, W, stim.pointer(),
+                                       stim.n, float(t_mid), n_long - 1, 0.01, 0.01,
+                                       states[1].data_ptr(), s.cuda_stream))
+    nb = L.fk_adjoint_workspace_bytes(H, W)
+    work = torch.empty(nb // 8 + 1, dtype=torch.float64, device=dev)
+    g = torch.Generator(device="cpu").manual_seed(1)
+    lam = [torch.randn(shape, generator=g).to(dev) * 1e-3 for _ in range(3)]
+    gD = torch.zeros(shape, device=dev)
+    _lib.check(L.fk_adjoint_prepare(Dd.data_ptr(), H, W, work.data_ptr(), nb, s.cuda_stream))
+
+    def seg(n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        _lib.check(L.fk_adjoint_segment(
+            ctypes.byref(p), Dd.data_ptr(), H, W, stim.pointer(), stim.n, float(t_mid), n,
+            0.01, 0.01, states.data_ptr(), None, lam[0].data_ptr(), lam[1].data_ptr(),
+            lam[2].data_ptr(), gD.data_ptr(), work.data_ptr(), nb, s.cuda_stream))
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    seg(n_long)  # warm-up
+    t_long = min(seg(n_long) for _ in range(3))
+    t_short = min(seg(n_short) for _ in range(3))
+    return (t_long - t_short) / (n_long - n_short)
 
 
 def bench_inverse(seed, n=32, iters=50, n_steps=1000, frame_every=50):

@@ -47,26 +47,27 @@ class _Rollout(torch.autograd.Function):
         L = _lib.lib()
         stim = _Stimuli(cfg["stimuli"], dev, (H, W))
         p = _lib.make_params(prm)
-        nb = L.fk_forward_workspace_bytes(H, W)
-        work = torch.empty(max(1, nb // 4), dtype=torch.float32, device=dev)
         s = stream_handle()
         frames = torch.empty((n // fe, H, W), dtype=torch.float32, device=dev)
+        # Each checkpoint interval runs through fk_forward_trajectory, the call
+        # the backward pass uses to recompute it, so the backward linearises
+        # around exactly these states (same kernels, same rounding), whatever
+        # step kernel is set or tuned.
+        seg = torch.empty((min(S, max(n, 1)), 3, H, W), dtype=torch.float32, device=dev)
         ckpts = {0: st}
         cur = st
         k = 0
         while k < n:
-            nxt = min(n, (k // S + 1) * S, (k // fe + 1) * fe)
-            out = State(*(torch.empty((H, W), dtype=torch.float32, device=dev)
-                          for _ in range(3)))
-            _lib.check(L.fk_forward_euler(
+            e = min(n, k + S)
+            _lib.check(L.fk_forward_trajectory(
                 ctypes.byref(p), D.data_ptr(), cur.v.data_ptr(), cur.w.data_ptr(),
-                cur.u.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + k), float(t0 + nxt),
-                float(cfg["dt"]), float(cfg["dx"]), out.v.data_ptr(), out.w.data_ptr(),
-                out.u.data_ptr(), work.data_ptr(), nb, s))
-            cur, k = out, nxt
-            if k % fe == 0:
-                frames[k // fe - 1].copy_(cur.u)
-            if k % S == 0 and k < n:
+                cur.u.data_ptr(), H, W, stim.pointer(), stim.n, float(t0 + k), e - k,
+                float(cfg["dt"]), float(cfg["dx"]), seg.data_ptr(), s))
+            for j in range((k // fe + 1) * fe, e + 1, fe):  # x_j is seg[j - k - 1]
+                frames[j // fe - 1].copy_(seg[j - k - 1, 2])
+            last = seg[e - k - 1]
+            cur, k = State(last[0].clone(), last[1].clone(), last[2].clone()), e
+            if k < n:
                 ckpts[k] = cur
         ctx.cfg = cfg
         ctx.prm = prm
@@ -128,12 +129,18 @@ def rollout(D: torch.Tensor, tau_d, tau_r, tau_si, state0, params, stimuli: Sequ
     fields of ``params``; pass floats for constants, which also spares the
     host a device synchronisation). Other fields of ``params``, the initial
     state and the stimuli are constants."""
-    dev = require_gpu("rollout")
     if int(frame_every) <= 0 or int(n_steps) < 0:
         raise ValueError("frame_every must be positive and n_steps non-negative")
+    shape = tuple(state0[2].shape)  # the u field of State(v, w, u)
+    if tuple(D.shape) != shape:
+        raise ValueError("D has shape %s, the state has %s" % (tuple(D.shape), shape))
+    dev = require_gpu("rollout")
     D = on_device(D, dev) if not isinstance(D, torch.Tensor) else D
     if D.dtype != torch.float32 or D.device != dev:
         raise ValueError("D must be a float32 tensor on the GPU")
+    # the kernels read D as a row-major H x W array; a strided view is copied
+    # (differentiably, so its gradient maps back onto the caller's view)
+    D = D.contiguous()
     taus, host = [], []
     for x in (tau_d, tau_r, tau_si):
         if isinstance(x, torch.Tensor):

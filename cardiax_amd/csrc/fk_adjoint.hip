@@ -324,6 +324,17 @@ __device__ __forceinline__ void axis_transpose(const float* __restrict__ Q2,
   }
 }
 
+// Index of x wave `xw` (0 or 1) of the current block of k_adj_main among all
+// x waves of its grid: one tau accumulator each.
+__device__ __forceinline__ size_t adj_wave_index(int xw) {
+  return ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 2 + (size_t)xw;
+}
+
+// Number of x waves (tau accumulators) of k_adj_main on an H x W grid.
+__host__ __device__ inline size_t adj_n_waves(int H, int W) {
+  return (size_t)((W + 63) / 64) * (size_t)((H + 1) / 2) * 2;
+}
+
 // One reverse step from the derived fields, two threads per cell in
 // different waves of a block of 64 columns x 2 rows: the "x" thread loads and
 // applies the transposed operators along axis 0, evaluates the reaction's
@@ -331,8 +342,10 @@ __device__ __forceinline__ void axis_transpose(const float* __restrict__ Q2,
 // (and, with NEXT, derives the axis-1 part of the previous step's fields) and
 // hands its 4 results over through LDS (one barrier). Twice the waves and half
 // the dependency chain per wave: the step is latency-bound on small grids.
-// Each cell adds its tau-gradient contribution to its own accumulator
-// g_tau[cell].{x, y, z} (deterministic; summed by k_adj_reduce).
+// The tau-gradient contributions of the 64 cells of an x wave are summed
+// across the wave (a fixed butterfly) and lane 0 adds the three sums to the
+// wave's own double accumulator g_tau[wave]: 24 bytes per 64 cells per step,
+// deterministic (each accumulator is touched by one wave, in step order).
 // NEXT: also write the derived fields of the previous step's reverse (they
 // depend on the new cotangent only at the same cell), so a segment needs one
 // launch per step.
@@ -342,7 +355,7 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
            const float* __restrict__ u, int H, int W, StimArgs sa, float dt, float inv_dx2,
            const float* __restrict__ lv, const float* __restrict__ lw, AdjPre pre,
            float* __restrict__ lv_o, float* __restrict__ lw_o, float* __restrict__ lu_o,
-           float* __restrict__ gD, float4* __restrict__ g_tau,
+           float* __restrict__ gD, double* __restrict__ g_tau,
            const AdjAxisTab* __restrict__ tab_g, AdjNext nx) {
   __shared__ AdjAxisTab tab[2];
   __shared__ float4 xch[kAdjThreads / 2];  // y thread -> x thread of each cell
@@ -359,7 +372,6 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
   float nD = 0.0f, nDx = 0.0f, nDy = 0.0f, inj = 0.0f;
   float vv = 0.0f, ww = 0.0f, uu = 0.0f, lvv = 0.0f, lww = 0.0f, L = 0.0f, lap = 0.0f;
   float sx = 0.0f, ex = 0.0f, gx_n = 0.0f, lapx_n = 0.0f;
-  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float gd = 0.0f;
   bool stim = false;
   if (ythread) {
@@ -396,54 +408,68 @@ k_adj_main(Coef c, const float* __restrict__ v, const float* __restrict__ w,
     lvv = lv[idx];
     lww = lw[idx];
     gd = gD[idx];
-    acc = g_tau[idx];
     for (int k = 0; k < sa.n; ++k) stim = stim || (sa.f[k][idx] != 0.0f);
     axis_transpose(pre.Q2, pre.QPx, idx, W, i, H, tab[0], sx, ex);
     if (NEXT) axis_pre(xs, i, H, tab[0], gx_n, lapx_n);
   }
   __syncthreads();  // every thread of the block passes this one barrier
-  if (ythread || !in) return;
-  const float4 yr = xch[q];
-  const float sy = yr.x, ey = yr.y;
+  if (ythread) return;  // whole waves: the x waves stay complete for the tau sums
+  float t_d = 0.0f, t_r = 0.0f, t_si = 0.0f;  // this cell's tau-gradient terms
+  if (in) {
+    const float4 yr = xch[q];
+    const float sy = yr.x, ey = yr.y;
 
-  // --- reaction, gates as fixed masks -----------------------------------
-  const bool p = uu >= c.V_c;
-  const float inv_tvm = (uu >= c.V_v) ? c.inv_tv2m : c.inv_tv1m;
-  const float dfv_dv = p ? -c.inv_tvp : -inv_tvm;
-  const float dfw_dw = p ? -c.inv_twp : -c.inv_twm;
-  const float S = half_one_plus_tanh(c, uu);  // (1 + tanh(k (u - V_csi))) / 2
-  // dS/du = 2k S (1 - S); kk = -2 k log2(e)
-  const float dS = -c.kk * 0.69314718055994531f * S * (1.0f - S);
-  // j_ion = -(j_fi + j_so + j_si) / Cm
-  const float djdv = p ? (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_cm : 0.0f;
-  const float djdw = S * c.inv_tsi * c.inv_cm;
-  const float djdu =
-      -((p ? -vv * (1.0f + c.V_c - 2.0f * uu) * c.inv_td : 0.0f) + (p ? 0.0f : c.inv_t0) -
-        ww * dS * c.inv_tsi) *
-      c.inv_cm;
-  // stimulus replaces j_ion: no cotangent reaches it where active
-  const float lus = stim ? 0.0f : L;
+    // --- reaction, gates as fixed masks -----------------------------------
+    const bool p = uu >= c.V_c;
+    const float inv_tvm = (uu >= c.V_v) ? c.inv_tv2m : c.inv_tv1m;
+    const float dfv_dv = p ? -c.inv_tvp : -inv_tvm;
+    const float dfw_dw = p ? -c.inv_twp : -c.inv_twm;
+    const float S = half_one_plus_tanh(c, uu);  // (1 + tanh(k (u - V_csi))) / 2
+    // dS/du = 2k S (1 - S); kk = -2 k log2(e)
+    const float dS = -c.kk * 0.69314718055994531f * S * (1.0f - S);
+    // j_ion = -(j_fi + j_so + j_si) / Cm
+    const float djdv = p ? (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_cm : 0.0f;
+    const float djdw = S * c.inv_tsi * c.inv_cm;
+    const float djdu =
+        -((p ? -vv * (1.0f + c.V_c - 2.0f * uu) * c.inv_td : 0.0f) + (p ? 0.0f : c.inv_t0) -
+          ww * dS * c.inv_tsi) *
+        c.inv_cm;
+    // stimulus replaces j_ion: no cotangent reaches it where active
+    const float lus = stim ? 0.0f : L;
 
-  lv_o[idx] = lvv + dt * (dfv_dv * lvv + djdv * lus);
-  lw_o[idx] = lww + dt * (dfw_dw * lww + djdw * lus);
-  const float lu_new = L + dt * (djdu * lus + (sx + sy) * inv_dx2);
-  lu_o[idx] = lu_new;
-  gD[idx] = gd + dt * (L * lap + ex + ey) * inv_dx2;
-  // tau gradients: d j_ion / d tau
-  const float djd_td = p ? -vv * (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_td * c.inv_cm
-                         : 0.0f;
-  const float djd_tr = p ? c.inv_tr * c.inv_tr * c.inv_cm : 0.0f;
-  const float djd_tsi = -ww * S * c.inv_tsi * c.inv_tsi * c.inv_cm;
-  acc.x += dt * djd_td * lus;
-  acc.y += dt * djd_tr * lus;
-  acc.z += dt * djd_tsi * lus;
-  g_tau[idx] = acc;
-  if (NEXT) {  // derived fields of the previous step, at this cell
-    const float Ln = lu_new + inj;
-    nx.pre.Llap[idx] = make_float2(Ln, lapx_n + yr.w);
-    nx.pre.QPx[idx] = make_float2(nDx * Ln, Ln * gx_n);
-    nx.pre.QPy[idx] = make_float2(nDy * Ln, Ln * yr.z);
-    nx.pre.Q2[idx] = nD * Ln;
+    lv_o[idx] = lvv + dt * (dfv_dv * lvv + djdv * lus);
+    lw_o[idx] = lww + dt * (dfw_dw * lww + djdw * lus);
+    const float lu_new = L + dt * (djdu * lus + (sx + sy) * inv_dx2);
+    lu_o[idx] = lu_new;
+    gD[idx] = gd + dt * (L * lap + ex + ey) * inv_dx2;
+    // tau gradients: d j_ion / d tau
+    const float djd_td = p ? -vv * (uu - c.V_c) * (1.0f - uu) * c.inv_td * c.inv_td * c.inv_cm
+                           : 0.0f;
+    const float djd_tr = p ? c.inv_tr * c.inv_tr * c.inv_cm : 0.0f;
+    const float djd_tsi = -ww * S * c.inv_tsi * c.inv_tsi * c.inv_cm;
+    t_d = dt * djd_td * lus;
+    t_r = dt * djd_tr * lus;
+    t_si = dt * djd_tsi * lus;
+    if (NEXT) {  // derived fields of the previous step, at this cell
+      const float Ln = lu_new + inj;
+      nx.pre.Llap[idx] = make_float2(Ln, lapx_n + yr.w);
+      nx.pre.QPx[idx] = make_float2(nDx * Ln, Ln * gx_n);
+      nx.pre.QPy[idx] = make_float2(nDy * Ln, Ln * yr.z);
+      nx.pre.Q2[idx] = nD * Ln;
+    }
+  }
+  // every lane of the x wave takes part; lanes off the grid add 0
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    t_d += __shfl_xor(t_d, o, 64);
+    t_r += __shfl_xor(t_r, o, 64);
+    t_si += __shfl_xor(t_si, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    double* a = g_tau + 3 * adj_wave_index(threadIdx.x >> 6);
+    a[0] += (double)t_d;
+    a[1] += (double)t_r;
+    a[2] += (double)t_si;
   }
 }
 
@@ -455,19 +481,19 @@ __global__ void k_adj_zero(float* a, size_t n) {
 
 constexpr int kRedBlocks = 512;
 
-// Sums of the three per-cell tau accumulators (x, y, z of n float4), in
-// double: each block reduces a grid-stride slice into part[block][0..2]
-// (k_adj_reduce), then one block adds the block sums (k_adj_reduce_final).
+// Sums of the three per-wave tau accumulators (n triples of doubles): each
+// block reduces a grid-stride slice into part[block][0..2] (k_adj_reduce),
+// then one block adds the block sums (k_adj_reduce_final). Fixed order:
+// deterministic.
 __global__ void __launch_bounds__(256)
-k_adj_reduce(const float4* __restrict__ acc, size_t n, double* __restrict__ part) {
+k_adj_reduce(const double* __restrict__ acc, size_t n, double* __restrict__ part) {
   __shared__ double s[3][256];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   for (size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x; b < n;
        b += (size_t)gridDim.x * blockDim.x) {
-    const float4 p = acc[b];
-    a0 += (double)p.x;
-    a1 += (double)p.y;
-    a2 += (double)p.z;
+    a0 += acc[3 * b];
+    a1 += acc[3 * b + 1];
+    a2 += acc[3 * b + 2];
   }
   s[0][threadIdx.x] = a0;
   s[1][threadIdx.x] = a1;
@@ -500,9 +526,9 @@ k_adj_reduce_final(const double* __restrict__ part, int n_part, float* g_tau) {
     for (int k = 0; k < 3; ++k) g_tau[k] = (float)s[k][0];
 }
 
-// Workspace layout: [tables (2 axes)][Dx][Dy][tau accumulators: float4 per cell][cotangent
-// ping-pong: 3 fields][derived fields: 2 sets of 7 (8 floats per cell)]
-// [tau block sums]
+// Workspace layout: [tables (2 axes)][Dx][Dy][tau accumulators: 3 doubles per x wave of
+// k_adj_main][cotangent ping-pong: 3 fields][derived fields: 2 sets of 7 (8 floats per
+// cell)][tau block sums]
 struct AdjLayout {
   size_t tab, dx, dy, part, lam, pre, red, total;
 };
@@ -519,7 +545,7 @@ static AdjLayout adj_layout(int H, int W) {
   L.tab = take(2 * sizeof(AdjAxisTab));
   L.dx = take(cells * sizeof(float));
   L.dy = take(cells * sizeof(float));
-  L.part = take(cells * sizeof(float4));
+  L.part = take(3 * adj_n_waves(H, W) * sizeof(double));
   L.lam = take(3 * cells * sizeof(float));
   L.pre = take(2 * 8 * cells * sizeof(float));
   L.red = take(3 * kRedBlocks * sizeof(double));
@@ -562,11 +588,11 @@ static hipError_t launch_main(const AdjLayout& L, char* wk, const Coef& c, const
   if (u_prev)
     k_adj_main<true><<<grid, kAdjThreads, 0, s>>>(
         c, v, w, u, H, W, sa, dt, inv_dx2, lam_in[0], lam_in[1], pre_fields(L, wk, set, n),
-        lam_out[0], lam_out[1], lam_out[2], gD, (float4*)(wk + L.part), tab, nx);
+        lam_out[0], lam_out[1], lam_out[2], gD, (double*)(wk + L.part), tab, nx);
   else
     k_adj_main<false><<<grid, kAdjThreads, 0, s>>>(
         c, v, w, u, H, W, sa, dt, inv_dx2, lam_in[0], lam_in[1], pre_fields(L, wk, set, n),
-        lam_out[0], lam_out[1], lam_out[2], gD, (float4*)(wk + L.part), tab, nx);
+        lam_out[0], lam_out[1], lam_out[2], gD, (double*)(wk + L.part), tab, nx);
   return hipGetLastError();
 }
 
@@ -598,7 +624,7 @@ int32_t fk_adjoint_prepare(const float* D, int32_t H, int32_t W, void* work, siz
   const dim3 block(64, 4), grid((W + 63) / 64, (H + 3) / 4);
   k_adj_dgrad<<<grid, block, 0, s>>>(D, H, W, tab, (float*)(wk + L.dx), (float*)(wk + L.dy));
   HIP_CHECK(hipGetLastError());
-  k_adj_zero<<<256, 256, 0, s>>>((float*)(wk + L.part), 4 * (size_t)H * W);
+  k_adj_zero<<<256, 256, 0, s>>>((float*)(wk + L.part), 6 * adj_n_waves(H, W));
   HIP_CHECK(hipGetLastError());
   return FK_OK;
 }
@@ -685,11 +711,11 @@ int32_t fk_adjoint_tau_grads(void* work, int32_t H, int32_t W, float* g_tau, voi
   ErrorScope es;
   if (!check_grid(H, W) || !check_ptrs({work, g_tau})) return last_code();
   const AdjLayout L = adj_layout(H, W);
-  const size_t n = (size_t)H * W;
+  const size_t n = adj_n_waves(H, W);
   const int blocks = (int)std::min<size_t>((n + 255) / 256, (size_t)kRedBlocks);
   double* part = (double*)((char*)work + L.red);
   k_adj_reduce<<<blocks, 256, 0, (hipStream_t)stream>>>(
-      (const float4*)((char*)work + L.part), n, part);
+      (const double*)((char*)work + L.part), n, part);
   k_adj_reduce_final<<<1, 256, 0, (hipStream_t)stream>>>(part, blocks, g_tau);
   HIP_CHECK(hipGetLastError());
   return FK_OK;

@@ -236,6 +236,13 @@ __global__ void k_fill(float* a, size_t n, float x) {
     a[i] = x;
 }
 
+// field = x on columns [0, c1) of every row, 0 elsewhere.
+__global__ void k_fill_cols(float* a, int H, int W, int c1, float x) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)H * W;
+       i += (size_t)gridDim.x * blockDim.x)
+    a[i] = (int)(i % (size_t)W) < c1 ? x : 0.0f;
+}
+
 __global__ void k_init(float* v, float* w, float* u, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -717,15 +724,31 @@ int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
   const size_t n = (size_t)H * W;
   const size_t wb = fk_forward_workspace_bytes(H, W);
   float* mem = nullptr;
-  HIP_CHECK(hipMalloc((void**)&mem, 7 * n * sizeof(float) + 256 + wb));
-  float *D = mem, *x = mem + n, *y = mem + 4 * n;
-  float* wk = (float*)(((uintptr_t)(mem + 7 * n) + 255) / 256 * 256);
+  HIP_CHECK(hipMalloc((void**)&mem, 8 * n * sizeof(float) + 256 + wb));
+  float *D = mem, *x = mem + n, *y = mem + 4 * n, *f = mem + 7 * n;
+  float* wk = (float*)(((uintptr_t)(mem + 8 * n) + 255) / 256 * 256);
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
   k_fill<<<blocks, 256, 0, s>>>(D, n, 1e-3f);
-  k_init<<<blocks, 256, 0, s>>>(x, x + n, x + 2 * n, (int64_t)n);
-  // PARAMSET_3 (cardiax/params.py:60-62); the kernels' cost does not depend on values
+  k_init<<<blocks, 256, 0, s>>>(y, y + n, y + 2 * n, (int64_t)n);
+  // PARAMSET_3 (cardiax/params.py:60-62)
   const fk_params p{3.33f, 19.6f, 1250.0f, 870.0f, 41.0f, 0.25f, 12.5f,
                     33.33f, 29.0f, 10.0f, 0.85f, 0.13f, 0.04f, 1.0f};
+  // The gates compile to branches, so a step costs more where waves hold
+  // excited or mixed cells than at rest. The candidates are timed from the
+  // state 200 steps after an S1 pulse over the left 40% of the columns: that
+  // part excited, the rest at rest, a wavefront between them. No stimulus is
+  // active while timing, as for all but a few steps of a real solve.
+  k_fill_cols<<<blocks, 256, 0, s>>>(f, H, W, (W * 2) / 5, 20.0f);
+  HIP_CHECK(hipGetLastError());
+  {
+    const fk_stimulus s1{0.0, 2.0, 1e9, f};
+    const int32_t rc = forward_loop(kStream2, &p, D, y, y + n, y + 2 * n, H, W, &s1, 1, 0.0,
+                                    200.0, 0.01f, 0.01f, x, x + n, x + 2 * n, wk, wb, s, false);
+    if (rc != FK_OK) {
+      (void)hipFree(mem);
+      return rc;
+    }
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);

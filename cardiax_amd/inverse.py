@@ -117,13 +117,7 @@ def recover(instances: Sequence[Instance], iters: int = 50, lr: float = 2e-5,
             s = streams[k % len(streams)]
             with torch.cuda.stream(s):
                 opt.zero_grad(set_to_none=True)
-                p = inst.params
-                D = prm.D().contiguous()
-                fr = adjoint.rollout(D, p.tau_d, p.tau_r, p.tau_si, inst.state0, p, inst.stimuli,
-                                     0, inst.n_steps, inst.frame_every, dt, dx)
-                diff = (fr - inst.target).double()
-                loss = (diff * diff).mean()  # float64 accumulation
-                loss.backward()
+                loss = _loss_backward(inst, prm.D().contiguous(), dt, dx)
                 opt.step()
                 with torch.no_grad():  # interpolation keeps D within the nodes' range
                     prm.theta.clamp_(D_MIN, D_MAX)
@@ -141,6 +135,37 @@ def recover(instances: Sequence[Instance], iters: int = 50, lr: float = 2e-5,
         err1.append(float(torch.linalg.norm(D.double() - ref)) / nrm)
     return {"D": Ds, "loss": losses.cpu().numpy(), "seconds": wall,
             "d_err_initial": err0, "d_err_final": err1}
+
+
+def _loss_backward(inst: Instance, D: torch.Tensor, dt: float, dx: float) -> torch.Tensor:
+    """Frame MSE of one instance at D (accumulated in float64) and its
+    backward pass through the adjoint: the gradient reaches whatever D was
+    computed from. Runs on the current stream."""
+    p = inst.params
+    fr = adjoint.rollout(D, p.tau_d, p.tau_r, p.tau_si, inst.state0, p, inst.stimuli, 0,
+                         inst.n_steps, inst.frame_every, dt, dx)
+    diff = (fr - inst.target).double()
+    loss = (diff * diff).mean()
+    loss.backward()
+    return loss
+
+
+def d_gradients(instances: Sequence[Instance], Ds: Sequence[torch.Tensor], n_streams: int = 8,
+                dt: float = wl.DT, dx: float = wl.DX) -> List[torch.Tensor]:
+    """The gradient of each instance's frame MSE with respect to its full D
+    field at Ds[k], by the pass ``recover`` makes in every iteration
+    (instances interleaved on ``n_streams`` HIP streams)."""
+    require_gpu("inverse")
+    streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
+    torch.cuda.synchronize()  # inputs made on other streams are complete
+    out = []
+    for k, (inst, D) in enumerate(zip(instances, Ds)):
+        with torch.cuda.stream(streams[k % len(streams)]):
+            Dk = D.detach().clone().requires_grad_(True)
+            _loss_backward(inst, Dk, dt, dx)
+            out.append(Dk.grad)
+    torch.cuda.synchronize()
+    return out
 
 
 def forward_only_seconds(instances: Sequence[Instance], n_streams: int = 8,

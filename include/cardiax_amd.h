@@ -93,8 +93,11 @@ int32_t fk_set_forward_variant(int32_t variant);
 
 /* Autotuner of the forward step kernel. No reference counterpart (the
  * reference leaves kernel choice to XLA). Times every variant except the
- * plain one over n_steps Euler steps (1..100000) of an H x W grid on `stream`
- * (synchronising it; scratch memory is allocated and freed here) and records
+ * plain one over n_steps Euler steps (1..100000) of an H x W grid on `stream`,
+ * from a state with an excited region and a wavefront (200 steps after an S1
+ * pulse over the left 40% of the columns; the branches of the gates make the
+ * cost depend on the state), synchronising the stream; scratch memory is
+ * allocated and freed here. It records
  * the fastest for that shape: automatic launches (variant 0) on an H x W grid
  * use it from then on, process-wide. times_ms (may be NULL) receives one time
  * per variant index (-1 for variants not timed); best (may be NULL) the

@@ -100,11 +100,21 @@ size_t FN(fk_oracle_scratch_bytes)(int H, int W) {
  * fields: n_stim packed (H, W) arrays; proto: n_stim x (start, duration,
  * period). scratch: FN(fk_oracle_scratch_bytes)(H, W) bytes. nthreads <= 0
  * keeps the OpenMP default. Returns the number of steps taken.
+ *
+ * gates / gate_mode: the Heaviside gates p = [u >= V_c], q = [u >= V_v] of
+ * every cell and step (one byte per cell-step, bit 0 = p, bit 1 = q, step s
+ * at gates + s * H * W). gate_mode 0 ignores `gates`; 1 records them; 2
+ * replays them in place of the comparisons. Replay makes the loop the smooth
+ * function of D, the time constants and the state that the adjoint
+ * differentiates (jax.grad treats greater_equal as constant), so central
+ * differences of it converge where those of the true loop jump wherever a
+ * gate flips.
  */
 int64_t FN(fk_oracle_forward)(const double* params, const REAL* D, REAL* v, REAL* w, REAL* u,
                               int H, int W, int n_stim, const double* proto,
                               const REAL* fields, double t, double t_end, double dt,
-                              double dx, void* scratch, int nthreads) {
+                              double dx, void* scratch, int nthreads, uint8_t* gates,
+                              int gate_mode) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -149,14 +159,20 @@ int64_t FN(fk_oracle_forward)(const double* params, const REAL* D, REAL* v, REAL
         if (pf[q] != 0) st[q] = pf[q];
     }
 
+    uint8_t* gs = gates ? gates + (size_t)s * H * W : NULL;
 #pragma omp parallel for schedule(static)
     for (int i = 1; i < PH - 1; ++i) {
       for (int j = 1; j < PW - 1; ++j) {
         const size_t q = (size_t)i * PW + j;
         const size_t o = (size_t)(i - 1) * W + (j - 1);
         const REAL vv = pv[q], ww = pw[q], uu = pu[q];
-        const REAL p = uu >= Vc ? (REAL)1 : (REAL)0;
-        const REAL qq = uu >= Vv ? (REAL)1 : (REAL)0;
+        REAL p = uu >= Vc ? (REAL)1 : (REAL)0;
+        REAL qq = uu >= Vv ? (REAL)1 : (REAL)0;
+        if (gate_mode == 1 && gs) gs[o] = (uint8_t)((p != 0) | ((qq != 0) << 1));
+        if (gate_mode == 2 && gs) {
+          p = (gs[o] & 1) ? (REAL)1 : (REAL)0;
+          qq = (gs[o] & 2) ? (REAL)1 : (REAL)0;
+        }
         const REAL tvm = ((REAL)1 - qq) * tv1m + qq * tv2m;
         const REAL jfi = -vv * p * (uu - Vc) * ((REAL)1 - uu) / td;
         const REAL jso = (uu * ((REAL)1 - p) / t0) + (p / tr);

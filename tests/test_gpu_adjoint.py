@@ -268,3 +268,60 @@ def test_segment_equals_step_by_step_reversal(gpu):
         assert torch.equal(a, b)
     assert torch.equal(gD, gD2)
     assert torch.equal(gt, gt2)
+
+
+@pytest.mark.parametrize("variant", ["auto", "streamc2", "tb64x2", "pair2"])
+def test_rollout_forward_is_the_recomputed_trajectory(gpu, variant):
+    """The frames rollout returns are the states the backward pass recomputes
+    (fk_forward_trajectory from the same checkpoint), bit for bit, whichever
+    step kernel is set: the adjoint linearises around the forward's own
+    trajectory."""
+    shape, n, fe, S = (40, 52), 30, 5, 7
+    st, Dstar, stims = _c4_small(shape, n, fe, seed=11)
+    prm = P.PARAMSET_3
+    H, W = shape
+    Lb = _lib.lib()
+    prev = Lb.fk_set_forward_variant(_lib.VARIANTS[variant])
+    try:
+        D = torch.from_numpy(Dstar).to(gpu)
+        with torch.no_grad():
+            fr = adjoint.rollout(D, prm.tau_d, prm.tau_r, prm.tau_si, st, prm, stims, 0, n, fe,
+                                 0.01, 0.01, checkpoint_every=S)
+        traj = torch.empty((n, 3, H, W), device=gpu)
+        x0 = [torch.from_numpy(a).to(gpu) for a in st]
+        stim = _dev_stims(stims, gpu, shape)
+        p = _lib.make_params(prm)
+        _lib.check(Lb.fk_forward_trajectory(ctypes.byref(p), D.data_ptr(), x0[0].data_ptr(),
+                                            x0[1].data_ptr(), x0[2].data_ptr(), H, W,
+                                            stim.pointer(), stim.n, 0.0, n, 0.01, 0.01,
+                                            traj.data_ptr(),
+                                            torch.cuda.current_stream().cuda_stream))
+        for j in range(fe, n + 1, fe):
+            assert torch.equal(fr[j // fe - 1], traj[j - 1, 2]), j
+    finally:
+        Lb.fk_set_forward_variant(prev)
+
+
+def test_rollout_strided_D_gets_its_gradient(gpu):
+    """A non-contiguous D (a column slice of a wider array) is read as the
+    H x W field it denotes, and its gradient lands on the view."""
+    shape, n, fe = (36, 30), 60, 20
+    st, Dstar, stims = _c4_small(shape, n, fe, seed=12)
+    prm = P.PARAMSET_3
+    target = torch.stack([x.u for x in solve.forward(st, list(range(0, n + 1, fe)), prm, Dstar,
+                                                     stims, 0.01, 0.01)])
+
+    def grad_of(Dt):
+        fr = adjoint.rollout(Dt, prm.tau_d, prm.tau_r, prm.tau_si, st, prm, stims, 0, n, fe,
+                             0.01, 0.01)
+        ((fr - target) ** 2).mean().backward()
+
+    D = torch.full(shape, 1e-3, device=gpu, requires_grad=True)
+    grad_of(D)
+    wide = torch.full((shape[0], shape[1] + 9), 1e-3, device=gpu, requires_grad=True)
+    view = wide[:, 4:4 + shape[1]]
+    assert not view.is_contiguous()
+    grad_of(view)
+    assert torch.equal(wide.grad[:, 4:4 + shape[1]], D.grad)
+    assert torch.count_nonzero(wide.grad[:, :4]) == 0
+    assert torch.count_nonzero(wide.grad[:, 4 + shape[1]:]) == 0

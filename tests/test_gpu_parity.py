@@ -210,22 +210,29 @@ def test_c1_parity_1000_steps(variant):
 
 
 def test_c1_activation_every_step(gpu):
-    """Activation maps at single-step resolution over 400 steps (S1 wave)."""
-    sim = wl.c1_inputs(0, steps=400)
+    """Activation maps at single-step resolution over all 1000 steps of C1:
+    the S1 wave and the S2 wave (S2 fires at step 640), first crossings of
+    u >= 0.5 within one step of the FP64 oracle's."""
+    sim = wl.c1_inputs(0, steps=1000)
+    s2 = sim.meta["s2_start"]
+    assert s2 < 1000
     stims = [(s.protocol, s.field) for s in sim.stimuli]
     ostims = [o.Stimulus(o.Protocol(*s.protocol), s.field) for s in sim.stimuli]
     g, r = solve.init(sim.shape), o.init(sim.shape)
-    gu, ru = [], []
-    for t in range(400):
+    ga = np.full(sim.shape, np.inf)
+    ra = np.full(sim.shape, np.inf)
+    for t in range(1000):
         g = solve._forward_euler(g, t, t + 1, sim.params, sim.D, stims, 0.01, 0.01)
         r = c_oracle.forward_euler(r, t, t + 1, sim.params, sim.D, ostims, 0.01, 0.01)
-        gu.append(npy(g.u))
-        ru.append(r.u)
-    ga, ra = _activation(gu), _activation(ru)
+        for act, u in ((ga, npy(g.u)), (ra, r.u)):
+            act[np.isinf(act) & (u >= 0.5)] = t + 1
     assert np.array_equal(np.isinf(ga), np.isinf(ra))
     fin = np.isfinite(ga)
     assert fin.sum() > 500
+    assert (ga[fin] > s2).sum() > 500  # cells first activated by the S2 wave
     assert np.abs(ga[fin] - ra[fin]).max() <= 1
+    print("C1 per-step activation: %d cells, %d after S2 (step %d), max |diff| %d steps"
+          % (fin.sum(), (ga[fin] > s2).sum(), s2, np.abs(ga[fin] - ra[fin]).max()))
 
 
 def test_forward_checkpoints_and_composition(gpu):

@@ -83,3 +83,49 @@ def test_c_segments_compose_and_zero_steps_is_identity():
     z = c_oracle.forward_euler(st, 5, 5, sim.params, sim.D, stims, 0.01, 0.01)
     for x, y in zip(z, st):
         assert np.array_equal(x, y)
+
+
+def test_gate_replay_reproduces_the_run_and_is_the_autograd_function():
+    """gate_mode 1 records the Heaviside gates, 2 replays them. Replaying the
+    masks of the run itself changes nothing (bitwise). With the masks held,
+    the loop is the smooth function that autograd (comparisons carry no
+    gradient) differentiates: central differences of it equal the FP64
+    autograd restatement's gradient."""
+    import torch
+
+    from oracle import fk_autograd as fa
+
+    shape, n, fe = (24, 20), 90, 30
+    rng = np.random.default_rng(4)
+    D0 = 1e-3 * (0.6 + 0.4 * rng.uniform(size=shape))
+    f1 = np.zeros(shape); f1[:, :4] = 20.0
+    stims = [(o.Protocol(0, 2, 1e9), f1)]
+    ost = [o.Stimulus(p, f) for p, f in stims]
+    prm = P.PARAMSET_3
+    target = rng.uniform(0, 1, (n // fe,) + shape)
+
+    def frames(D, gates=None, mode=0):
+        x, out = o.init(shape), []
+        for t in range(0, n, fe):
+            g = gates[t:t + fe] if gates is not None else None
+            x = c_oracle.forward_euler(x, t, t + fe, prm, D, ost, 0.01, 0.01, gates=g,
+                                       gate_mode=mode)
+            out.append(x.u)
+        return np.stack(out)
+
+    G = np.zeros((n,) + shape, np.uint8)
+    a = frames(D0, G, 1)
+    assert np.array_equal(a, frames(D0))
+    assert np.array_equal(a, frames(D0, G, 2))
+    assert 0 < (G & 1).mean() < 1  # both sides of the gate occur
+    Dt = torch.tensor(D0, requires_grad=True)
+    fr = fa.rollout_frames(o.init(shape), prm, Dt, stims, 0, n, fe, 0.01, 0.01)
+    ((fr - torch.tensor(target)) ** 2).mean().backward()
+    g = Dt.grad.numpy()
+    for trial in range(3):
+        dD = rng.normal(size=shape)
+        h = 1e-9
+        lp = np.mean((frames(D0 + h * dD, G, 2) - target) ** 2)
+        lm = np.mean((frames(D0 - h * dD, G, 2) - target) ** 2)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - float((g * dD).sum())) <= 1e-6 * abs(fd), (trial, fd)
