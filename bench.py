@@ -33,7 +33,9 @@ KERNELS = {"stream8": ("k_stream<8, true>", 1), "stream16": ("k_stream<16, true>
            "streamc16": ("k_stream_c<16>", 1), "stream4": ("k_stream<4, true>", 1),
            "streamc4": ("k_stream_c<4>", 1), "stream2": ("k_stream<2, true>", 1),
            "streamc2": ("k_stream_c<2>", 1), "pair2": ("k_stream2<2, true>", 1),
-           "pair4": ("k_stream2<4, true>", 1)}
+           "pair4": ("k_stream2<4, true>", 1), "streamt8": ("k_stream_t<8>", 2),
+           "streamt12": ("k_stream_t<12>", 2), "streamt16": ("k_stream_t<16>", 2),
+           "streamt24": ("k_stream_t<24>", 2)}
 
 
 def log(*a):
@@ -552,7 +554,13 @@ def main():
 
     tuned = None
     if not args.no_tune:  # setup, outside the timed region
-        tuned = autotune.tune((H, W), n_steps=300)
+        # on the workload's own state after S2 (the spiral regime that fills
+        # most of the run), with its parameters, D-map and stimuli
+        t_mid = min(sim.t1 - 300, sim.meta["s2_start"] + (sim.t1 - sim.meta["s2_start"]) // 4)
+        mid = solve._forward_euler(st0, sim.t0, t_mid, sim.params, D, stims, sim.dt, sim.dx)
+        tuned = autotune.tune_on(mid, t_mid, sim.params, D, stims, sim.dt, sim.dx, n_steps=300)
+        tuned["state_at_step"] = t_mid
+        del mid
         log("autotune %dx%d: %s" % (H, W, json.dumps(tuned)))
     variant = autotune.resolved((H, W))
 

@@ -29,7 +29,8 @@ FK_NOALLOC_MAX_CELLS = 1 << 20
 VARIANTS = {"auto": 0, "plain": 1, "tile16": 2, "tile32": 3, "stream8": 4, "stream16": 5,
             "stream32": 6, "tb64x2": 7, "tb64x4": 8, "tb32x2": 9, "tb32x4": 10,
             "streamc8": 11, "streamc16": 12, "stream4": 13, "streamc4": 14,
-            "stream2": 15, "streamc2": 16, "pair2": 17, "pair4": 18}
+            "stream2": 15, "streamc2": 16, "pair2": 17, "pair4": 18, "streamt8": 19,
+            "streamt12": 20, "streamt16": 21, "streamt24": 22}
 
 c_f = ctypes.c_float
 c_d = ctypes.c_double
@@ -67,6 +68,10 @@ _SIGNATURES = {
     "fk_set_forward_variant": (c_i32, [c_i32]),
     "fk_autotune_forward": (c_i32, [c_i32, c_i32, c_i32, c_vp, ctypes.POINTER(c_f),
                                     ctypes.POINTER(c_i32)]),
+    "fk_autotune_forward_state": (c_i32, [ctypes.POINTER(FkParams), c_vp, c_vp, c_vp, c_vp,
+                                          c_i32, c_i32, ctypes.POINTER(FkStimulus), c_i32, c_d,
+                                          c_i32, c_f, c_f, c_vp, ctypes.POINTER(c_f),
+                                          ctypes.POINTER(c_i32)]),
     "fk_autotune_lookup": (c_i32, [c_i32, c_i32]),
     "fk_forward_variant_for": (c_i32, [c_i32, c_i32]),
     "fk_autotune_reset": (None, []),

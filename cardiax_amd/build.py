@@ -34,6 +34,12 @@ HIPCC_FLAGS = [
     "-Wno-unused-function",
 ]
 
+# Per-file additions. The two-step streaming kernels fully unroll band loops of
+# several thousand instructions, past LLVM's default size limit for
+# `#pragma unroll`; a loop left rolled would index its register windows at run
+# time and move them to scratch memory.
+FILE_FLAGS = {"fk_forward.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"]}
+
 # The host pass does not know the packed-fp32 feature and says so.
 _NOISE = "'-packed-fp32-ops' is not a recognized feature"
 
@@ -55,7 +61,7 @@ def build(verbose: bool = False) -> str:
     fd, tmp = tempfile.mkstemp(prefix=".libcardiax_amd.", suffix=".so.tmp", dir=HERE)
     os.close(fd)
     objs = [os.path.join(tmpdir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
-    cmds = [[hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c",
+    cmds = [[hipcc(), *HIPCC_FLAGS, *FILE_FLAGS.get(s, []), "-I", INCLUDE, "-I", CSRC, "-c",
              os.path.join(CSRC, s), "-o", o] for s, o in zip(SOURCES, objs)]
     try:
         workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))

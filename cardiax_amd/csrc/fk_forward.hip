@@ -26,6 +26,7 @@
 #include "fk_host.h"
 #include "fk_stream.h"
 #include "fk_stream2.h"
+#include "fk_stream_t.h"
 #include "fk_tb.h"
 
 namespace fk {
@@ -320,7 +321,11 @@ enum Variant : int {
   kStreamC2 = 16,   // ... with precomputed coefficients
   kPair2 = 17,      // streaming bands of 2 / 4 rows, two adjacent columns per lane
   kPair4 = 18,      // (W odd: the one-column kernels 15 / 13)
-  kNumVariants = 19
+  kStreamT8 = 19,   // two steps per launch on streaming bands of 8 / 12 / 16 / 24
+  kStreamT12 = 20,  // rows (grids too small for them: 15)
+  kStreamT16 = 21,
+  kStreamT24 = 22,
+  kNumVariants = 23
 };
 
 static bool is_pc(int v) {
@@ -335,6 +340,43 @@ static int pc_base(int v) {
 }
 
 static bool is_tb(int v) { return v >= kTb64x2 && v <= kTb32x4; }
+
+static bool is_two_step(int v) { return v >= kStreamT8 && v <= kStreamT24; }
+
+// Whether the two-step variant v fits an H x W grid (else stream2 runs).
+static bool two_step_fits(int v, int H, int W) {
+  switch (v) {
+    case kStreamT8: return stream_t_fits<8>(H, W);
+    case kStreamT12: return stream_t_fits<12>(H, W);
+    case kStreamT16: return stream_t_fits<16>(H, W);
+    default: return stream_t_fits<24>(H, W);
+  }
+}
+
+template <int R>
+static void launch_stream_t(const Coef& c, const float* D, const float* v, const float* w,
+                            const float* u, int H, int W, const StimArgs& s1,
+                            const StimArgs& s2, float dt, float inv_dx2, float* ov, float* ow,
+                            float* ou, hipStream_t s) {
+  const StreamTGeom g = make_stream_t_geom<R>(H, W);
+  k_stream_t<R><<<g.blocks, 256, 0, s>>>(c, D, v, w, u, H, W, s1, s2, dt, inv_dx2, ov, ow, ou,
+                                         g);
+}
+
+// Two Euler steps (times t and t+1) with the two-step variant v.
+static hipError_t launch_two_step(int v, const Coef& c, const float* D, const float* x0,
+                                  const float* x1, const float* x2, int H, int W,
+                                  const StimArgs& s1, const StimArgs& s2, float dt,
+                                  float inv_dx2, float* ov, float* ow, float* ou,
+                                  hipStream_t s) {
+  switch (v) {
+    case kStreamT8: launch_stream_t<8>(c, D, x0, x1, x2, H, W, s1, s2, dt, inv_dx2, ov, ow, ou, s); break;
+    case kStreamT12: launch_stream_t<12>(c, D, x0, x1, x2, H, W, s1, s2, dt, inv_dx2, ov, ow, ou, s); break;
+    case kStreamT16: launch_stream_t<16>(c, D, x0, x1, x2, H, W, s1, s2, dt, inv_dx2, ov, ow, ou, s); break;
+    default: launch_stream_t<24>(c, D, x0, x1, x2, H, W, s1, s2, dt, inv_dx2, ov, ow, ou, s); break;
+  }
+  return hipGetLastError();
+}
 
 static std::atomic<int> g_variant{kAuto};
 
@@ -353,14 +395,19 @@ static int tuned_variant(int H, int W) {
 }
 
 // Default when no variant is set and the shape was not tuned, from the
-// variants measured on MI355X: streaming bands of 2 rows up to 1.5 M cells
-// and of 4 rows up to 8 M (small bands give enough waves to hide load latency
-// while the grid sits in the 256 MB MALL), and the LDS tile of 16 rows above
-// (grids in DRAM, where the bands' halo re-reads cost DRAM bandwidth and a
-// tile stages its halo once).
+// variants measured on MI355X. Up to 1.1 M cells, streaming bands of 2 rows:
+// small bands give enough waves to hide latency, and a grid this small is
+// limited by issue rate more than by memory. Above, two steps per launch
+// (bands of 8 rows up to 2.5 M cells, of 16 above): half the memory traffic
+// and half the launches per step, 20-30% faster at 1200^2, 2048^2 and
+// 4096^2. Grids too small for those run 2-row bands up to 8 M cells and the
+// LDS tile above.
 static int default_variant(int H, int W) {
   const size_t n = (size_t)H * W;
-  return n <= 1536 * 1024 ? (int)kStream2 : n <= 8 * 1024 * 1024 ? (int)kStream4 : (int)kTile16;
+  if (n <= 1100 * 1024) return kStream2;
+  if (n <= 2560 * 1024 && stream_t_fits<8>(H, W)) return kStreamT8;
+  if (stream_t_fits<16>(H, W)) return kStreamT16;
+  return n <= 8 * 1024 * 1024 ? (int)kStream4 : (int)kTile16;
 }
 static int resolve_variant(int v) { return v == kAuto ? (int)kStream2 : v; }
 
@@ -458,6 +505,10 @@ static hipError_t launch_step(int variant, bool euler, const Coef& c, const floa
       if (W % 2 == 0) launch_stream2<4>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s);
       else launch_stream<4>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s);
       break;
+    case kStreamT8:  // single steps of the two-step variants: the 2-row bands
+    case kStreamT12:
+    case kStreamT16:
+    case kStreamT24: launch_stream<2>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
     case kStream8:
     default: launch_stream<8>(euler, c, D, v, w, u, H, W, sa, dt, inv_dx2, ov, ow, ou, s); break;
   }
@@ -496,7 +547,7 @@ static hipError_t launch_tb_variant(int variant, const Coef& c, const float* D, 
 static hipError_t launch_heun(int stage, const Coef& c, const float* D, const float* const x[3],
                               int H, int W, const StimArgs& sa, float dt, float inv_dx2,
                               float* const o[3], const HeunIO& io, hipStream_t s) {
-  const bool two = default_variant(H, W) == kStream2;
+  const bool two = (size_t)H * W <= 1536 * 1024;  // 2-row bands up to 1.5 M cells
   const StreamGeom g = two ? make_stream_geom<2>(H, W) : make_stream_geom<4>(H, W);
 #define FK_HEUN(R, ST)                                                                     \
   k_stream_heun<R, ST><<<g.blocks, 256, 0, s>>>(c, D, x[0], x[1], x[2], H, W, sa, dt, inv_dx2, \
@@ -573,6 +624,7 @@ int32_t forward_loop(int variant, const fk_params* params, const float* D, const
   int v = resolve_for(variant, H, W);
   // nothing to amortise, or no room for the coefficients
   if (is_pc(v) && (steps < 2 || state_only)) v = pc_base(v);
+  if (is_two_step(v) && !two_step_fits(v, H, W)) v = kStream2;
   float4* cf = nullptr;
   if (is_pc(v)) {
     cf = (float4*)((char*)work + coef_offset(H, W));
@@ -580,7 +632,7 @@ int32_t forward_loop(int variant, const fk_params* params, const float* D, const
     k_stream_coef<<<grid, 256, 0, s>>>(D, H, W, (float)(inv_dx2 / 144.0), cf);
     HIP_CHECK(hipGetLastError());
   }
-  const int depth = is_tb(v) ? tb_depth(v) : 1;
+  const int depth = is_tb(v) ? tb_depth(v) : is_two_step(v) ? 2 : 1;
   const int64_t launches = (steps + depth - 1) / depth;
   // Ping-pong so that the last launch lands in the output buffers.
   int dst = (launches % 2 == 1) ? 0 : 1;
@@ -591,7 +643,18 @@ int32_t forward_loop(int variant, const fk_params* params, const float* D, const
   for (int i = 0; i < n_stimuli; ++i) st.f[i] = stimuli[i].field;
   for (int64_t k = 0; k < steps; k += depth) {
     float** d = bufs[dst];
-    if (is_tb(v)) {
+    if (is_two_step(v)) {
+      const int ns = (int)std::min<int64_t>(2, steps - k);
+      const StimArgs s1 = sched.at(t + (double)k);  // copies: at() reuses its storage
+      if (ns == 2) {
+        const StimArgs s2 = sched.at(t + (double)(k + 1));
+        HIP_CHECK(launch_two_step(v, c, D, src[0], src[1], src[2], H, W, s1, s2, dt, inv_dx2,
+                                  d[0], d[1], d[2], s));
+      } else {  // an odd last step
+        HIP_CHECK(launch_step(kStream2, true, c, D, src[0], src[1], src[2], H, W, s1, dt,
+                              inv_dx2, d[0], d[1], d[2], s));
+      }
+    } else if (is_tb(v)) {
       const int ns = (int)std::min<int64_t>(depth, steps - k);
       for (int j = 0; j < 8; ++j) st.act[j] = j < ns ? sched.mask(t + (double)(k + j)) : 0u;
       HIP_CHECK(launch_tb_variant(v, c, D, src[0], src[1], src[2], H, W, st, ns, dt, inv_dx2,
@@ -617,6 +680,57 @@ int32_t forward_loop(int variant, const fk_params* params, const float* D, const
     src[2] = d[2];
     dst ^= 1;
   }
+  return FK_OK;
+}
+
+// Times every variant but the plain one over n_steps Euler steps from
+// (v, w, u) at time t with the given problem (HIP events on `s`, which is
+// synchronised): a warm-up of 8 steps, then the timed run, results into
+// y (3 H W floats); the inputs are not modified. times_ms[variant] receives
+// each time (-1 for the untimed), *best the fastest.
+static int32_t time_candidates(const fk_params* p, const float* D, const float* v,
+                               const float* w, const float* u, int H, int W,
+                               const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                               int32_t n_steps, float dt, float dx, float* y, void* wk,
+                               size_t wb, hipStream_t s, float* times_ms, int32_t* best) {
+  const size_t n = (size_t)H * W;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int best_v = default_variant(H, W);
+  float best_ms = 0.0f;
+  const int cand[] = {kStream8,  kStream4,  kStream2,   kStream16,  kStream32,  kStreamC8,
+                      kStreamC4, kStreamC2, kStreamC16, kPair2,     kPair4,     kTile16,
+                      kTile32,   kTb64x2,   kTb64x4,    kTb32x2,    kTb32x4,    kStreamT8,
+                      kStreamT12, kStreamT16, kStreamT24};
+  if (times_ms)
+    for (int k = 0; k < kNumVariants; ++k) times_ms[k] = -1.0f;
+  int32_t rc = FK_OK;
+  for (int var : cand) {
+    if (e != hipSuccess) break;
+    // warm-up launch, then the timed run
+    rc = forward_loop(var, p, D, v, w, u, H, W, stimuli, n_stimuli, t, t + 8.0, dt, dx, y,
+                      y + n, y + 2 * n, wk, wb, s, false);
+    if (rc == FK_OK) e = hipEventRecord(e0, s);
+    if (rc == FK_OK && e == hipSuccess)
+      rc = forward_loop(var, p, D, v, w, u, H, W, stimuli, n_stimuli, t, t + (double)n_steps,
+                        dt, dx, y, y + n, y + 2 * n, wk, wb, s, false);
+    if (rc == FK_OK && e == hipSuccess) e = hipEventRecord(e1, s);
+    if (rc == FK_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (rc == FK_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (rc != FK_OK || e != hipSuccess) break;
+    if (times_ms) times_ms[var] = ms;
+    if (var == cand[0] || ms < best_ms) {
+      best_ms = ms;
+      best_v = var;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc != FK_OK) return rc;
+  if (e != hipSuccess) return set_error(FK_ERR_HIP, "autotune failed: %s", hipGetErrorString(e));
+  *best = best_v;
   return FK_OK;
 }
 
@@ -733,59 +847,59 @@ int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
   // PARAMSET_3 (cardiax/params.py:60-62)
   const fk_params p{3.33f, 19.6f, 1250.0f, 870.0f, 41.0f, 0.25f, 12.5f,
                     33.33f, 29.0f, 10.0f, 0.85f, 0.13f, 0.04f, 1.0f};
-  // The gates compile to branches, so a step costs more where waves hold
-  // excited or mixed cells than at rest. The candidates are timed from the
-  // state 200 steps after an S1 pulse over the left 40% of the columns: that
-  // part excited, the rest at rest, a wavefront between them. No stimulus is
-  // active while timing, as for all but a few steps of a real solve.
+  // The gates compile to branches in some kernels, so a step costs more where
+  // waves hold excited or mixed cells than at rest. The candidates are timed
+  // from the state 200 steps after an S1 pulse over the left 40% of the
+  // columns: that part excited, the rest at rest, a wavefront between them.
+  // No stimulus is active while timing, as for all but a few steps of a real
+  // solve. fk_autotune_forward_state times on the caller's own problem.
   k_fill_cols<<<blocks, 256, 0, s>>>(f, H, W, (W * 2) / 5, 20.0f);
   HIP_CHECK(hipGetLastError());
+  int32_t rc;
   {
     const fk_stimulus s1{0.0, 2.0, 1e9, f};
-    const int32_t rc = forward_loop(kStream2, &p, D, y, y + n, y + 2 * n, H, W, &s1, 1, 0.0,
-                                    200.0, 0.01f, 0.01f, x, x + n, x + 2 * n, wk, wb, s, false);
-    if (rc != FK_OK) {
-      (void)hipFree(mem);
-      return rc;
-    }
+    rc = forward_loop(kStream2, &p, D, y, y + n, y + 2 * n, H, W, &s1, 1, 0.0, 200.0, 0.01f,
+                      0.01f, x, x + n, x + 2 * n, wk, wb, s, false);
   }
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipError_t e = hipEventCreate(&e0);
-  if (e == hipSuccess) e = hipEventCreate(&e1);
-  int best_v = default_variant(H, W);
-  float best_ms = 0.0f;
-  const int cand[] = {kStream8,  kStream4,  kStream2,  kStream16, kStream32, kStreamC8,
-                      kStreamC4, kStreamC2, kStreamC16, kPair2,   kPair4,    kTile16,
-                      kTile32,   kTb64x2,   kTb64x4,   kTb32x2,  kTb32x4};
-  if (times_ms)
-    for (int k = 0; k < kNumVariants; ++k) times_ms[k] = -1.0f;
-  for (int v : cand) {
-    if (e != hipSuccess) break;
-    // warm-up launch, then the timed run
-    int32_t rc = forward_loop(v, &p, D, x, x + n, x + 2 * n, H, W, nullptr, 0, 0.0, 8.0, 0.01f,
-                              0.01f, y, y + n, y + 2 * n, wk, wb, s, false);
-    if (rc == FK_OK) e = hipEventRecord(e0, s);
-    if (rc == FK_OK && e == hipSuccess)
-      rc = forward_loop(v, &p, D, x, x + n, x + 2 * n, H, W, nullptr, 0, 0.0, (double)n_steps,
-                        0.01f, 0.01f, y, y + n, y + 2 * n, wk, wb, s, false);
-    if (rc == FK_OK && e == hipSuccess) e = hipEventRecord(e1, s);
-    if (rc == FK_OK && e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0.0f;
-    if (rc == FK_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (rc != FK_OK || e != hipSuccess) break;
-    if (times_ms) times_ms[v] = ms;
-    if (v == cand[0] || ms < best_ms) {
-      best_ms = ms;
-      best_v = v;
-    }
-  }
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
+  int32_t best_v = 0;
+  if (rc == FK_OK)
+    rc = time_candidates(&p, D, x, x + n, x + 2 * n, H, W, nullptr, 0, 0.0, n_steps, 0.01f,
+                         0.01f, y, wk, wb, s, times_ms, &best_v);
   const hipError_t ef = hipFree(mem);
-  if (e == hipSuccess) e = ef;
-  if (last_code() != FK_OK) return last_code();
-  if (e != hipSuccess)
-    return set_error(FK_ERR_HIP, "autotune failed: %s", hipGetErrorString(e));
+  if (rc != FK_OK) return rc;
+  if (ef != hipSuccess) return set_error(FK_ERR_HIP, "autotune failed: %s", hipGetErrorString(ef));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[{H, W}] = best_v;
+  }
+  if (best) *best = best_v;
+  return FK_OK;
+}
+
+int32_t fk_autotune_forward_state(const fk_params* params, const float* D, const float* v,
+                                  const float* w, const float* u, int32_t H, int32_t W,
+                                  const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                                  int32_t n_steps, float dt, float dx, void* stream,
+                                  float* times_ms, int32_t* best) {
+  ErrorScope es;
+  if (!check_params(params) || !check_grid(H, W) || !check_stimuli(stimuli, n_stimuli) ||
+      !check_ptrs({D, v, w, u}) || !check_scalar("dt", dt) || !check_dx(dx))
+    return last_code();
+  if (n_steps < 1 || n_steps > 100000)
+    return set_error(FK_ERR_ARG, "n_steps must be in [1, 100000], got %d", n_steps);
+  if (!check_time(t, t + (double)n_steps)) return last_code();
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)H * W;
+  const size_t wb = fk_forward_workspace_bytes(H, W);
+  float* mem = nullptr;
+  HIP_CHECK(hipMalloc((void**)&mem, 3 * n * sizeof(float) + 256 + wb));
+  float* wk = (float*)(((uintptr_t)(mem + 3 * n) + 255) / 256 * 256);
+  int32_t best_v = 0;
+  const int32_t rc = time_candidates(params, D, v, w, u, H, W, stimuli, n_stimuli, t, n_steps,
+                                     dt, dx, mem, wk, wb, s, times_ms, &best_v);
+  const hipError_t ef = hipFree(mem);
+  if (rc != FK_OK) return rc;
+  if (ef != hipSuccess) return set_error(FK_ERR_HIP, "autotune failed: %s", hipGetErrorString(ef));
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[{H, W}] = best_v;
@@ -807,6 +921,7 @@ int32_t fk_forward_variant_for(int32_t H, int32_t W) {
   const int v = resolve_for(g_variant.load(), H, W);
   if (W % 2 && v == kPair2) return kStream2;
   if (W % 2 && v == kPair4) return kStream4;
+  if (is_two_step(v) && !two_step_fits(v, H, W)) return kStream2;
   return v;
 }
 

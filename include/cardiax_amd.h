@@ -75,7 +75,8 @@ int32_t fk_abi_version(void);
  * choice to XLA). Process-wide. Returns the previous setting, or FK_ERR_ARG
  * for an unknown variant.
  *    0      automatic: the autotuned choice for the grid shape, else 15 up to
- *           1.5 M cells, 13 up to 8 M cells and 2 above
+ *           1.1 M cells, 19 up to 2.5 M cells and 21 above (grids too small
+ *           for those: 15, 13 up to 8 M cells, 2 above)
  *    1      plain, one thread per cell
  *    2, 3   LDS tile of 16 / 32 rows
  *    4-6    streaming wavefront bands of 8 / 16 / 32 rows
@@ -87,8 +88,11 @@ int32_t fk_abi_version(void);
  *    14, 16 the same with precomputed coefficients
  *    17, 18 streaming bands of 2 / 4 rows with two adjacent columns per lane
  *           (for odd W: 15 / 13)
+ *    19-22  two Euler steps per launch on streaming bands of 8 / 12 / 16 / 24
+ *           rows, bit-identical to two steps of 15 (grids below 2R + 16 rows
+ *           or 112 columns run 15; an odd last step runs 15)
  * fk_step and fk_step_euler run 4 in place of 7-11, 5 in place of 12, 13 in
- * place of 14 and 15 in place of 16. */
+ * place of 14 and 15 in place of 16 and 19-22. */
 int32_t fk_set_forward_variant(int32_t variant);
 
 /* Autotuner of the forward step kernel. No reference counterpart (the
@@ -104,6 +108,20 @@ int32_t fk_set_forward_variant(int32_t variant);
  * chosen variant. */
 int32_t fk_autotune_forward(int32_t H, int32_t W, int32_t n_steps, void* stream,
                             float* times_ms, int32_t* best);
+
+/* The autotuner on the caller's own problem. No reference counterpart. Times
+ * every variant but the plain one over n_steps Euler steps (1..100000) from
+ * the state (v, w, u) at time t with the given parameters, D-map, stimuli, dt
+ * and dx, and records the fastest for the H x W shape as fk_autotune_forward
+ * does. The inputs are not modified; scratch memory is allocated and freed
+ * here, and `stream` is synchronised. Which side of the Heaviside gates the
+ * cells take decides how much work the branching kernels skip, so a state
+ * from the run to be made (a spiral, say) picks the kernel for that run. */
+int32_t fk_autotune_forward_state(const fk_params* params, const float* D, const float* v,
+                                  const float* w, const float* u, int32_t H, int32_t W,
+                                  const fk_stimulus* stimuli, int32_t n_stimuli, double t,
+                                  int32_t n_steps, float dt, float dx, void* stream,
+                                  float* times_ms, int32_t* best);
 
 /* The variant tuned for an H x W grid, or 0 when that shape was not tuned. */
 int32_t fk_autotune_lookup(int32_t H, int32_t W);

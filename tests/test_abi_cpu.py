@@ -51,11 +51,16 @@ def test_default_variant_by_grid_size():
     L = _lib.lib()
     prev = L.fk_set_forward_variant(0)
     try:
-        assert autotune.resolved((1024, 1024)) == "stream2"  # up to 1.5 M cells
-        assert autotune.resolved((1536, 1024)) == "stream2"
-        assert autotune.resolved((2048, 2048)) == "stream4"  # up to 8 M cells
-        assert autotune.resolved((2048, 4096)) == "stream4"
-        assert autotune.resolved((4096, 4096)) == "tile16"   # DRAM-resident grids
+        assert autotune.resolved((1024, 1024)) == "stream2"  # up to 1.1 M cells
+        assert autotune.resolved((1200, 1200)) == "streamt8"  # two steps per launch
+        assert autotune.resolved((2048, 2048)) == "streamt16"
+        assert autotune.resolved((4096, 4096)) == "streamt16"
+        assert autotune.resolved((40000, 100)) == "stream4"  # too narrow for two steps
+        assert autotune.resolved((100000, 100)) == "tile16"
+        L.fk_set_forward_variant(_lib.VARIANTS["streamt12"])
+        assert autotune.resolved((1024, 1024)) == "streamt12"
+        assert autotune.resolved((39, 1024)) == "stream2"  # below 2R + 16 rows
+        assert autotune.resolved((1024, 111)) == "stream2"  # below 112 columns
         L.fk_set_forward_variant(_lib.VARIANTS["pair2"])
         assert autotune.resolved((64, 64)) == "pair2"
         assert autotune.resolved((64, 63)) == "stream2"      # odd W: the one-column kernel

@@ -40,7 +40,8 @@ def test_forward_autotune_records_and_is_used(gpu):
         timed = set(res["times_ms"])
         assert timed == {"stream2", "stream4", "stream8", "stream16", "stream32", "streamc2",
                          "streamc4", "streamc8", "streamc16", "pair2", "pair4", "tile16",
-                         "tile32", "tb64x2", "tb64x4", "tb32x2", "tb32x4"}
+                         "tile32", "tb64x2", "tb64x4", "tb32x2", "tb32x4", "streamt8",
+                         "streamt12", "streamt16", "streamt24"}
         assert all(t > 0 for t in res["times_ms"].values())
         # the library takes the strict minimum of the unrounded times; the
         # reported times are rounded to 1e-4 ms, so near-ties may reorder
@@ -98,3 +99,29 @@ def test_batch_band_heights_match_per_simulation_runs(gpu, rows, use_graph):
                 assert torch.equal(a, r)
         with pytest.raises(ValueError):
             b.set_rows(12)
+
+
+def test_autotune_on_the_callers_problem(gpu):
+    """fk_autotune_forward_state times every candidate on the given state,
+    parameters, D-map and stimuli, leaves them unmodified, and records the
+    fastest for the shape like fk_autotune_forward."""
+    shape = (130, 140)
+    sim = wl.c1_inputs(4, shape=shape, steps=400)
+    stims = [(s.protocol._replace(start=0 if k == 0 else 150), torch.from_numpy(s.field).cuda())
+             for k, s in enumerate(sim.stimuli)]
+    D = torch.from_numpy(sim.D).cuda()
+    st = solve._forward_euler(solve.init(shape), 0, 200, sim.params, D, stims, 0.01, 0.01)
+    keep = [x.clone() for x in st]
+    autotune.reset()
+    try:
+        res = autotune.tune_on(st, 200, sim.params, D, stims, 0.01, 0.01, n_steps=30)
+        assert set(res["times_ms"]) == set(_lib.VARIANTS) - {"auto", "plain"}
+        assert all(t > 0 for t in res["times_ms"].values())
+        assert res["times_ms"][res["best"]] <= min(res["times_ms"].values()) + 1e-4
+        assert autotune.lookup(shape) == res["best"]
+        for a, b in zip(st, keep):
+            assert torch.equal(a, b)
+        with pytest.raises(ValueError):  # a D-map of another shape
+            autotune.tune_on(st, 200, sim.params, D[:, :-1], stims, 0.01, 0.01, n_steps=5)
+    finally:
+        autotune.reset()

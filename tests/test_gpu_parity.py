@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 KERNEL_VARIANTS = ["plain", "tile16", "tile32", "stream8", "stream16", "stream32",
                    "tb64x2", "tb64x4", "tb32x2", "tb32x4", "streamc8", "streamc16", "stream4",
-                   "streamc4", "stream2", "streamc2", "pair2", "pair4"]
+                   "streamc4", "stream2", "streamc2", "pair2", "pair4", "streamt8", "streamt12",
+                   "streamt16", "streamt24"]
 
 
 @pytest.fixture(params=KERNEL_VARIANTS)

@@ -79,7 +79,7 @@ def test_two_calls_second_smaller(gpu, pinned):
     _compare((96, 80), 2, 3, 34, gpu)      # a smaller one: odd step count
 
 
-@pytest.mark.parametrize("variant", ["stream4", "tb64x2", "pair2"])
+@pytest.mark.parametrize("variant", ["stream4", "tb64x2", "pair2", "streamt8"])
 def test_variants(gpu, variant):
     L = _lib.lib()
     prev = L.fk_set_forward_variant(_lib.VARIANTS[variant])
