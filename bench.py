@@ -172,6 +172,9 @@ def bench_batched(n_sims, n_steps, seed, per_sim_steps=64):
         n_bad = sum(int(not bool(torch.isfinite(x.u).all().item())) for x in b.states)
         umax = max(float(x.u.abs().max().item()) for x in b.states)
     batched = cells * n_steps / el / 1e9
+    # the tuned work list: one or two Euler steps per launch
+    per_launch = 2 if tuned["two_step"] else 1
+    kernel = ("k_batch_step_t<%d>" if tuned["two_step"] else "k_batch_step<%d>") % tuned["rows"]
     # per-simulation launches: the same simulations, one call each
     for x in bs[:4]:
         solve._forward_euler(x.state, 0, 4, x.params, x.diffusivity, x.stimuli, x.dt, x.dx)
@@ -191,8 +194,8 @@ def bench_batched(n_sims, n_steps, seed, per_sim_steps=64):
             "speedup_vs_per_sim": round(batched / per_sim, 2), "autotune": tuned,
             "u_finite": n_bad == 0,
             "n_nonfinite_sims": n_bad, "u_absmax": round(umax, 4),
-            "roofline": leg_roofline(BYTES_PER_CELL_UPDATE * cells, el / n_steps,
-                                     "k_batch_step<%d>" % tuned["rows"],
+            "roofline": leg_roofline(BYTES_PER_CELL_UPDATE * cells * per_launch,
+                                     el / (n_steps / per_launch), kernel,
                                      "%d simulations, %.1f M cells" % (n_sims, cells / 1e6))}
 
 

@@ -51,6 +51,8 @@ class FkStimulus(ctypes.Structure):
 
 
 FK_BATCH_MAX_STIMULI = 4
+FK_BATCH_N_OPTIONS = 6
+FK_BATCH_TWO_STEP = 256
 
 
 class FkSim(ctypes.Structure):

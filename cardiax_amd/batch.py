@@ -4,7 +4,8 @@ Each simulation is what one ``cardiax.solve._forward_euler`` call advances
 (cardiax/solve.py:92-100): its own state, parameter set, D-map, stimuli and
 time. ``Batch`` builds one work list over all of them; every Euler step of
 the whole batch is one kernel launch, and the launches are replayed from a
-recorded hipGraph (one graph launch per 32 steps). The reference has no
+recorded hipGraph (one graph launch per 32 steps); with a two-step option
+each launch advances the batch two steps. The reference has no
 batched path: its experiments loop over simulations one call at a time
 (experiments/generate_fd_data_128.py:25-56).
 """
@@ -20,6 +21,8 @@ from ._device import on_device, require_gpu, stream_handle
 from .solve import State
 
 FK_BATCH_MAX_STIMULI = _lib.FK_BATCH_MAX_STIMULI
+# fk_batch_autotune's option order
+BATCH_OPTIONS = ("2", "4", "8", "16", "t8", "t16")
 FkSim = _lib.FkSim
 
 
@@ -95,22 +98,33 @@ class Batch:
         return int(_lib.lib().fk_batch_n_items(self._h))
 
     def autotune(self, n_steps: int = 64) -> dict:
-        """Time the batched step with 2-, 4-, 8- and 16-row bands and keep the
-        fastest for later runs (fk_batch_autotune); the simulations' states are
-        not touched. Returns ``{"rows": R, "times_ms": [t2, t4, t8, t16]}``."""
+        """Time every available work-list option (bands of 2, 4, 8 or 16 rows
+        advanced one step per launch; bands of 8 or 16 rows advanced two steps
+        per launch when every simulation fits) and keep the fastest for later
+        runs (fk_batch_autotune); the simulations' states are not touched.
+        Returns ``{"rows": R, "two_step": bool, "times_ms": {option: ms}}``
+        with options "2", "4", "8", "16", "t8", "t16" (unavailable ones left
+        out)."""
         if self._h is None:
             raise RuntimeError("batch is closed")
-        times = (ctypes.c_float * 4)()
+        times = (ctypes.c_float * _lib.FK_BATCH_N_OPTIONS)()
         rows = ctypes.c_int32(0)
         _lib.check(_lib.lib().fk_batch_autotune(self._h, int(n_steps), stream_handle(), times,
                                             ctypes.byref(rows)))
-        return {"rows": int(rows.value), "times_ms": [round(float(t), 4) for t in times]}
+        r = int(rows.value)
+        return {"rows": r & ~_lib.FK_BATCH_TWO_STEP,
+                "two_step": bool(r & _lib.FK_BATCH_TWO_STEP),
+                "times_ms": {name: round(float(t), 4)
+                             for name, t in zip(BATCH_OPTIONS, times) if t >= 0}}
 
-    def set_rows(self, rows: int) -> None:
-        """Pin the band height (2, 4, 8 or 16) of later runs (fk_batch_set_rows)."""
+    def set_rows(self, rows: int, two_step: bool = False) -> None:
+        """Pin the work-list option of later runs (fk_batch_set_rows): a band
+        height of 2, 4, 8 or 16 rows, or 8 or 16 with ``two_step``
+        (NotImplementedError when some simulation does not fit it)."""
         if self._h is None:
             raise RuntimeError("batch is closed")
-        _lib.check(_lib.lib().fk_batch_set_rows(self._h, int(rows)))
+        flag = _lib.FK_BATCH_TWO_STEP if two_step else 0
+        _lib.check(_lib.lib().fk_batch_set_rows(self._h, int(rows) | flag))
 
     def run(self, n_steps: int, use_graph: bool = True) -> List[State]:
         """Advance all simulations by ``n_steps`` Euler steps (asynchronous on

@@ -38,7 +38,8 @@ HIPCC_FLAGS = [
 # several thousand instructions, past LLVM's default size limit for
 # `#pragma unroll`; a loop left rolled would index its register windows at run
 # time and move them to scratch memory.
-FILE_FLAGS = {"fk_forward.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"]}
+FILE_FLAGS = {"fk_forward.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"],
+              "fk_batch.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"]}
 
 # The host pass does not know the packed-fp32 feature and says so.
 _NOISE = "'-packed-fp32-ops' is not a recognized feature"

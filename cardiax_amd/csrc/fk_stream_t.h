@@ -30,7 +30,7 @@
 // one-sided rules apply where the reference applies them, and level 1's rows
 // beyond the grid are copies of its edge row (edge padding of the state at
 // t+1). In the first and last strip the lanes beyond the grid take the edge
-// column's level-1 value the same way. Requires H >= 2R + 16, W >= 112.
+// column's level-1 value the same way. Sizes: stream_t_fits.
 #pragma once
 
 #include "fk_stream.h"
@@ -53,9 +53,13 @@ inline StreamTGeom make_stream_t_geom(int H, int W) {
   return g;
 }
 
+// Grids the kernel handles: every band the first, the last or interior (H >=
+// 2R + 16), and every strip the first, the last or interior: two strips for
+// 56 <= W <= 96 (the last one's left halo inside the grid), three or more
+// from W = 104.
 template <int R>
 __host__ __device__ inline bool stream_t_fits(int H, int W) {
-  return H >= 2 * R + 16 && W >= 2 * kTOut + 16;
+  return H >= 2 * R + 16 && ((W >= 56 && W <= 2 * kTOut) || W >= 2 * kTOut + 8);
 }
 
 // Origin of tile k of n along an axis of N cells with tiles of T: the first
@@ -100,13 +104,14 @@ __device__ __forceinline__ float jion_only(const Coef& c, float v, float w, floa
 
 // XE: 0 interior band, 1 first band (rs = 0), 2 last band (rs = H - R).
 // YG: the strip touches a side (general column rules; lanes beyond the grid
-// padded after level 1). STIM: some stimulus is active at t or t+1.
-template <int R, int XE, bool YG, bool STIM>
+// padded after level 1). STIM: some stimulus is active at t or t+1. s1, s2:
+// the stimuli of the steps at t and t+1 (count(), on(m), field(m)).
+template <int R, int XE, bool YG, bool STIM, class SV>
 __device__ __forceinline__ void stream_t_tile(const Coef& c, const float* __restrict__ D,
                                               const float* __restrict__ v,
                                               const float* __restrict__ w,
                                               const float* __restrict__ u, int H, int W,
-                                              const StimArgs& s1, const StimArgs& s2, float dt,
+                                              const SV& s1, const SV& s2, float dt,
                                               float inv_dx2, float* __restrict__ o_v,
                                               float* __restrict__ o_w, float* __restrict__ o_u,
                                               int rs, int cs) {
@@ -193,8 +198,9 @@ __device__ __forceinline__ void stream_t_tile(const Coef& c, const float* __rest
     reaction_sel(c, v1[n], w1[n], uc, dv, dw, jion);
     if (STIM) {
       float sval = 0.0f;
-      for (int q = 0; q < s2.n; ++q) {
-        const float f = ldk(s2.f[q], n + 8);
+      for (int q = 0; q < s2.count(); ++q) {
+        if (!s2.on(q)) continue;
+        const float f = ldk(s2.field(q), n + 8);
         sval = (f != 0.0f) ? f : sval;
       }
       jion = (sval != 0.0f) ? sval : jion;
@@ -254,8 +260,9 @@ __device__ __forceinline__ void stream_t_tile(const Coef& c, const float* __rest
         jion = jion_only(c, vv, ww, uc);
       if (STIM) {
         float sval = 0.0f;
-        for (int q = 0; q < s1.n; ++q) {
-          const float f = ldk(s1.f[q], k);
+        for (int q = 0; q < s1.count(); ++q) {
+          if (!s1.on(q)) continue;
+          const float f = ldk(s1.field(q), k);
           sval = (f != 0.0f) ? f : sval;
         }
         jion = (sval != 0.0f) ? sval : jion;
@@ -315,8 +322,10 @@ k_stream_t(Coef c, const float* __restrict__ D, const float* __restrict__ v,
   const int xe = rs == 0 ? 1 : (rs == H - R ? 2 : 0);
   const bool yg = cs == 0 || cs == W - kTOut;
   const int sel = xe * 4 + (yg ? 2 : 0) + ((s1.n | s2.n) ? 1 : 0);
-#define FK_T(X, Y, S) \
-  stream_t_tile<R, X, Y, S>(c, D, v, w, u, H, W, s1, s2, dt, inv_dx2, o_v, o_w, o_u, rs, cs)
+  const StimListView v1{s1}, v2{s2};
+#define FK_T(X, Y, S)                                                                      \
+  stream_t_tile<R, X, Y, S, StimListView>(c, D, v, w, u, H, W, v1, v2, dt, inv_dx2, o_v, o_w, \
+                                          o_u, rs, cs)
   switch (sel) {
     case 0: FK_T(0, false, false); break;
     case 1: FK_T(0, false, true); break;

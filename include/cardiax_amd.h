@@ -312,6 +312,14 @@ int32_t fk_adjoint_tau_grads(void* work, int32_t H, int32_t W, float* g_tau, voi
 /* Most stimuli one simulation of a batch may carry. */
 #define FK_BATCH_MAX_STIMULI 4
 
+/* Work-list options of a batch: bands of 2, 4, 8 or 16 rows advanced one step
+ * per launch, and bands of 8 or 16 rows advanced two steps per launch (those
+ * two only when every simulation has >= 2R + 16 rows and 56..96 or >= 104
+ * columns). A band height or'ed with FK_BATCH_TWO_STEP names a two-step
+ * option. */
+#define FK_BATCH_N_OPTIONS 6
+#define FK_BATCH_TWO_STEP 256
+
 /* One simulation of a batch: its own constants, D-map, state (updated in
  * place by fk_batch_run), stimuli, time of the next step and steps. */
 typedef struct fk_sim {
@@ -336,22 +344,28 @@ typedef struct fk_batch fk_batch; /* opaque */
 int32_t fk_batch_create(const fk_sim* sims, int32_t n, fk_batch** out);
 
 /* Advances every simulation by n_steps Euler steps; results land in each
- * fk_sim's v, w, u and each simulation's time advances by n_steps.
+ * fk_sim's v, w, u and each simulation's time advances by n_steps. Each
+ * launch advances the whole batch one step (or two, with a two-step option);
+ * the stimulus activity of up to 32 steps comes from one small kernel.
  * use_graph != 0 replays a recorded hipGraph of 32 steps (one graph launch
- * per 32 steps for the whole batch), otherwise one launch per step. */
+ * per 32 steps for the whole batch), otherwise direct launches. */
 int32_t fk_batch_run(fk_batch* b, int64_t n_steps, int32_t use_graph, void* stream);
 
-/* Autotuner of the batch: times n_steps (1..10000) batched steps with bands
- * of 2, 4, 8 and 16 rows on `stream` (synchronising it) and keeps the fastest
- * work list for later runs (8 rows until tuned). Overwrites only the batch's
- * internal ping-pong pools, never a simulation's state. times_ms (4 floats,
- * may be NULL) and best_rows (may be NULL) receive the timings and the chosen
- * band height. */
+/* Autotuner of the batch: times n_steps (1..10000) batched steps with each
+ * available work-list option on `stream` (synchronising it) and keeps the
+ * fastest for later runs (8 rows, one step per launch, until tuned).
+ * Overwrites only the batch's internal ping-pong pools, never a simulation's
+ * state. times_ms (FK_BATCH_N_OPTIONS floats in the order 2, 4, 8, 16 rows,
+ * then 8, 16 rows with two steps; -1 for an unavailable option; may be NULL)
+ * and best_rows (may be NULL) receive the timings and the chosen band height
+ * (or'ed with FK_BATCH_TWO_STEP for a two-step option). */
 int32_t fk_batch_autotune(fk_batch* b, int32_t n_steps, void* stream, float* times_ms,
                           int32_t* best_rows);
 
-/* Pins the band height (2, 4, 8 or 16 rows) of later runs, as the autotuner
- * would; waits for the device first. */
+/* Pins the work-list option of later runs, as the autotuner would: a band
+ * height of 2, 4, 8 or 16 rows, or 8 or 16 or'ed with FK_BATCH_TWO_STEP
+ * (FK_ERR_UNSUPPORTED when some simulation does not fit two steps per
+ * launch); waits for the device first. */
 int32_t fk_batch_set_rows(fk_batch* b, int32_t rows);
 
 /* Number of (simulation, strip, band) tiles of one batch step. */

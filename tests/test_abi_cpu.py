@@ -57,10 +57,14 @@ def test_default_variant_by_grid_size():
         assert autotune.resolved((4096, 4096)) == "streamt16"
         assert autotune.resolved((40000, 100)) == "stream4"  # too narrow for two steps
         assert autotune.resolved((100000, 100)) == "tile16"
+        assert autotune.resolved((20000, 64)) == "streamt8"
         L.fk_set_forward_variant(_lib.VARIANTS["streamt12"])
         assert autotune.resolved((1024, 1024)) == "streamt12"
         assert autotune.resolved((39, 1024)) == "stream2"  # below 2R + 16 rows
-        assert autotune.resolved((1024, 111)) == "stream2"  # below 112 columns
+        assert autotune.resolved((1024, 100)) == "stream2"  # 97..103 columns do not fit
+        assert autotune.resolved((1024, 55)) == "stream2"   # below 56 columns
+        assert autotune.resolved((1024, 64)) == "streamt12"  # two strips
+        assert autotune.resolved((1024, 104)) == "streamt12"
         L.fk_set_forward_variant(_lib.VARIANTS["pair2"])
         assert autotune.resolved((64, 64)) == "pair2"
         assert autotune.resolved((64, 63)) == "stream2"      # odd W: the one-column kernel

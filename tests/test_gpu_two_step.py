@@ -3,7 +3,8 @@ against the one-step streaming kernel, bit for bit.
 
 Both compute every cell with the same operations in the same order, so two
 steps in one launch must equal two launches exactly. The shapes cover the
-smallest grids the kernels accept (2R + 16 rows, 112 columns), odd sides,
+smallest grids the kernels accept (2R + 16 rows; 56 to 96 columns in two
+strips, or 104 and more), odd sides,
 grids where the middle bands and strips overlap, and 1024^2; the stimuli are
 active on the first and on the second step of a launch, touch the edges and
 repeat; the parameter sets include the MAXFLOAT sentinel sets; the step
@@ -51,12 +52,14 @@ def _inputs(shape, seed):
 
 
 @pytest.mark.parametrize("variant", sorted(ROWS))
-@pytest.mark.parametrize("shape_kind", ["min", "min_odd", "overlap", "wide"])
+@pytest.mark.parametrize("shape_kind", ["min", "min_odd", "overlap", "wide", "two_strips",
+                                        "two_strips_wide"])
 @pytest.mark.parametrize("pname", ["3", "2", "7"])
 def test_two_steps_equal_one_step_kernel(gpu, variant, shape_kind, pname):
     R = ROWS[variant]
-    shape = {"min": (2 * R + 16, 112), "min_odd": (2 * R + 17, 113),
-             "overlap": (5 * R + 3, 3 * 48 + 21), "wide": (97, 517)}[shape_kind]
+    shape = {"min": (2 * R + 16, 104), "min_odd": (2 * R + 17, 113),
+             "overlap": (5 * R + 3, 3 * 48 + 21), "wide": (97, 517),
+             "two_strips": (2 * R + 19, 56), "two_strips_wide": (3 * R + 5, 96)}[shape_kind]
     st, D, stims = _inputs(shape, seed=R + len(shape_kind) + int(pname))
     prm = P.ALL_PARAMSETS[pname]
     for t0, t1 in ((1, 22), (2, 22)):  # 21 steps (odd) and 20
@@ -86,8 +89,9 @@ def test_small_grids_fall_back(gpu):
     prev = L.fk_set_forward_variant(_lib.VARIANTS["streamt24"])
     try:
         assert L.fk_forward_variant_for(63, 512) == _lib.VARIANTS["stream2"]  # < 2R + 16 rows
-        assert L.fk_forward_variant_for(64, 111) == _lib.VARIANTS["stream2"]  # < 112 columns
-        assert L.fk_forward_variant_for(64, 112) == _lib.VARIANTS["streamt24"]
+        assert L.fk_forward_variant_for(64, 100) == _lib.VARIANTS["stream2"]  # 97..103 columns
+        assert L.fk_forward_variant_for(64, 55) == _lib.VARIANTS["stream2"]
+        assert L.fk_forward_variant_for(64, 104) == _lib.VARIANTS["streamt24"]
     finally:
         L.fk_set_forward_variant(prev)
     st, D, stims = _inputs((40, 60), 1)
