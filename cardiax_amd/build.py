@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcardiax_amd.so")
 
 SOURCES = ["fk_common.hip", "fk_forward.hip", "fk_batch.hip", "fk_adjoint.hip", "fk_io.hip",
-           "fk_stimulus.hip", "fk_noalloc.hip"]
+           "fk_stimulus.hip", "fk_noalloc.hip", "fk_scar.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

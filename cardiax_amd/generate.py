@@ -11,9 +11,9 @@ linear and triangular directions are drawn from {NORTH, EAST, SOUTH} (never
 WEST), the coverage and angle come from the same key as the direction,
 ``random_sequence`` reuses the last stimulus key for the D-map, and an
 empty integer range returns its lower bound as ``jax.random.randint`` does.
-The D-map comes from the seeded scar generator of ``workloads.scar_map``
-(the reference's spline scars, deepx/utils_scars.py:402-426, need its PRNG)
-rescaled onto ``domain`` as the reference rescales its own.
+The D-map is the reference's spline scar (deepx/utils_scars.py:246-426,
+``scars.random_diffusivity``): contours on the host, rasterisation, blur and
+rescale onto ``domain`` on the GPU.
 
 The reference runs ``_forward_euler`` per checkpoint interval, pulls each
 state to the host and resizes it there with ``jax.image.resize`` before
@@ -29,29 +29,9 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import convert, io, solve, stimulus, workloads
+from . import convert, io, scars, solve, stimulus
 from ._device import require_gpu
-
-
-def _seq(rng) -> np.random.SeedSequence:
-    if isinstance(rng, np.random.SeedSequence):
-        return rng
-    if isinstance(rng, np.random.Generator):
-        return np.random.SeedSequence(int(rng.integers(0, 2 ** 63)))
-    return np.random.SeedSequence(int(rng))
-
-
-def _split(rng, n):
-    """jax.random.split: n independent child generators."""
-    return [np.random.default_rng(c) for c in _seq(rng).spawn(n)]
-
-
-def _randint(g, lo, hi, size=None):
-    """jax.random.randint(key, size, lo, hi): [lo, hi), lo when hi <= lo."""
-    lo, hi = int(lo), int(hi)
-    if hi <= lo:
-        return lo if size is None else np.full(size, lo, dtype=np.int64)
-    return int(g.integers(lo, hi)) if size is None else g.integers(lo, hi, size=size)
+from ._keys import _randint, _seq, _split  # noqa: F401  (jax.random stand-ins)
 
 
 def random_protocol(rng, min_start=0, max_start=1000, min_period=400, max_period=1e9):
@@ -101,12 +81,10 @@ def random_stimulus(rng, shape, min_start=0, max_start=0):
 
 
 def random_diffusivity(rng, shape, domain=(0.0001, 0.001)):
-    """deepx/generate.py:79-83 -- a scar map rescaled onto ``domain``
-    (cardiax/convert.py:23-26): its minimum becomes domain[0], its maximum
-    domain[1]."""
-    g = rng if isinstance(rng, np.random.Generator) else np.random.default_rng(_seq(rng))
-    c = workloads.scar_map(tuple(shape), g).astype(np.float64)
-    return convert.diffusivity_rescale(c, domain).astype(np.float32)
+    """deepx/generate.py:79-83 -- the random spline scar, inverted and
+    rescaled onto ``domain`` (cardiax/convert.py:23-26): its minimum becomes
+    domain[0], its maximum domain[1]. A float32 tensor on the current GPU."""
+    return scars.random_diffusivity(rng, tuple(int(x) for x in shape), domain)
 
 
 def random_sequence(rng, params, filepath, shape=(1200, 1200), n_stimuli=2, start=0,

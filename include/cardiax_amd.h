@@ -417,6 +417,49 @@ int32_t fk_stimulus_box(float* field, int32_t H, int32_t W, int32_t r0, int32_t 
 int32_t fk_affine_transform(const float* src, int32_t H, int32_t W, const double* matrix,
                             const double* offset, float* dst, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * The random scar D-map of the dataset generator (deepx/generate.py:79-83,
+ * deepx/utils_scars.py:246-426): its rasterisation and its blur.
+ * ------------------------------------------------------------------------- */
+
+/* Device scratch of fk_rasterize_polygons for n_polys polygons with
+ * n_vertices vertices in all. */
+size_t fk_rasterize_workspace_bytes(int32_t n_polys, int32_t n_vertices);
+
+/* deepx/utils_scars.py:79-97,377-381 (drawpolyintoemptycanvas via
+ * skimage.draw.polygon, the blobs summed and clipped to 1): mask (H x W,
+ * device) = 1 at every pixel (r, c) inside one of the n_polys closed
+ * polygons, 0 elsewhere. Polygon p has the vertices (rows[k], cols[k]) for
+ * k in [starts[p], starts[p+1]) (host arrays, >= 3 vertices each, read
+ * before the call returns). A pixel is inside when it lies in the polygon's
+ * box [int(max(0, min row)), ceil(max row)] x [int(max(0, min col)),
+ * ceil(max col)] and the crossing-number test at its centre holds: edge
+ * (i, j) counts when (row_i <= r < row_j or row_j <= r < row_i) and
+ * c < (col_j - col_i)(r - row_i) / (row_j - row_i) + col_i, in double.
+ * Pixels outside the image are dropped. work: fk_rasterize_workspace_bytes
+ * of device memory. Waits for `stream` before returning. */
+int32_t fk_rasterize_polygons(float* mask, int32_t H, int32_t W, const double* rows,
+                              const double* cols, const int32_t* starts, int32_t n_polys,
+                              void* work, size_t work_bytes, void* stream);
+
+/* deepx/utils_scars.py:402-406 (blur_scar: jax.scipy.signal.convolve2d(scar,
+ * kernel, "same")): dst = the "same"-size 2-D convolution of the H x W src
+ * with the kh x kw kernel (device, double, row-major):
+ * dst[i, j] = sum_{p, q} kernel[p, q] src[i + (kh-1)/2 - p, j + (kw-1)/2 - q]
+ * with integer halves and zero outside the image; products and sums in
+ * double, rounded once to float. dst must not be src. Stream-ordered. */
+int32_t fk_convolve_same(const float* src, int32_t H, int32_t W, const double* kernel,
+                         int32_t kh, int32_t kw, float* dst, void* stream);
+
+/* deepx/utils_scars.py:426 with deepx/generate.py:79-83 and
+ * cardiax/convert.py:23-26: D = diffusivity_rescale(1 - scar, (lo, hi)), i.e.
+ * c = 1 - scar, D = (c - min c)(hi - lo) / (max c - min c) + lo, in double,
+ * rounded once to float (n_cells cells, device). work: 4096 bytes of device
+ * memory (the min and max partials). A scar of one value gives NaN, as the
+ * reference's division by zero does. Stream-ordered. */
+int32_t fk_scar_to_diffusivity(const float* scar, int64_t n_cells, double lo, double hi,
+                               float* D, void* work, size_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,16 +64,6 @@ def test_stimulus_builders_all_drawn_and_start_honoured():
     assert np.array_equal(a, b)
 
 
-def test_diffusivity_rescaled_onto_domain():
-    for seed in range(5):
-        D = G.random_diffusivity(seed, (64, 80))
-        assert D.shape == (64, 80) and D.dtype == np.float32
-        assert D.min() == np.float32(1e-4) and D.max() == np.float32(1e-3)
-        assert np.array_equal(D, G.random_diffusivity(seed, (64, 80)))
-    D = G.random_diffusivity(0, (32, 32), domain=(2e-4, 5e-4))
-    assert D.min() == np.float32(2e-4) and D.max() == np.float32(5e-4)
-
-
 def test_random_sequence_refuses_without_gpu():
     from cardiax_amd.params import PARAMSET_3
 
