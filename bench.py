@@ -493,6 +493,81 @@ def bench_sequence(seed, n_steps, size=1200, every=500, reshape=256):
             "frames_finite": bool(np.isfinite(frames).all())}
 
 
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64, TFLOP/s
+
+
+def bench_scar(seed, size=1200, reps=5):
+    """deepx/generate.py:79-83 (random_diffusivity: the spline scar of
+    deepx/utils_scars.py:246-426) at the reference's default 1200^2 image:
+    the whole call (host contours + device rasterisation, blur and rescale)
+    and each device kernel on its own, timed with HIP events on the current
+    stream. The blur dominates: 2 kh kw FP64 flops per pixel (kh = kw =
+    int(0.1 size)), against the FP64 peak. The CPU leg is the oracle
+    pipeline (brute-force crossing numbers, scipy's FFT convolution, FP64
+    rescale) on one core."""
+    import numpy as np
+    import torch
+
+    from cardiax_amd import generate, scars
+
+    shape = (size, size)
+    generate.random_diffusivity(seed, shape)  # warm-up (library load, kernels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        D = generate.random_diffusivity(seed + r, shape)
+    torch.cuda.synchronize()
+    t_call = (time.perf_counter() - t0) / reps
+    g1, g2, ksize, sigma = scars._blur_draws(seed, shape)
+    t0 = time.perf_counter()
+    cs = scars.blob_contours(g2, shape, scars.DEF_PARAMS, scars.spline_centroids(g1))
+    t_host = time.perf_counter() - t0
+    k = scars.gauss_kernel(ksize, sigma)
+    mask = scars.rasterize(cs, shape)
+    blurred = scars.convolve_same(mask, k)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    stream = torch.cuda.current_stream()
+    n = 20
+    ev[0].record(stream)
+    for _ in range(n):
+        scars.convolve_same(mask, k)
+    ev[1].record(stream)
+    for _ in range(n):
+        scars.scar_to_diffusivity(blurred)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    blur_s = ev[0].elapsed_time(ev[1]) / 1e3 / n
+    rescale_s = ev[1].elapsed_time(ev[2]) / 1e3 / n
+    t0 = time.perf_counter()
+    for _ in range(n):
+        scars.rasterize(cs, shape, out=mask)  # synchronises per call
+    raster_s = (time.perf_counter() - t0) / n
+    flops = 2.0 * ksize * ksize * size * size
+    achieved = flops / blur_s / 1e12
+    # CPU: the oracle pipeline on the same contours
+    from oracle import scar_oracle as so
+
+    t0 = time.perf_counter()
+    m = so.union_mask(cs, shape)
+    t_mask = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = so.convolve_same_fft(m, so.gauss_kernel(ksize, sigma)).astype(np.float32)
+    so.scar_to_diffusivity(b)
+    t_fft = time.perf_counter() - t0
+    return {"workload": "random_diffusivity %dx%d: %d blobs, %dx%d Gaussian blur (sigma %.1f)"
+                        % (size, size, len(cs), ksize, ksize, sigma),
+            "ms_per_map": round(t_call * 1e3, 3), "host_contours_ms": round(t_host * 1e3, 3),
+            "raster_ms_incl_sync": round(raster_s * 1e3, 3),
+            "blur_ms": round(blur_s * 1e3, 3), "rescale_us": round(rescale_s * 1e6, 2),
+            "roofline": {"bound": "fp64", "achieved": round(achieved, 2),
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                         "kernel": "k_convolve_same", "flops_per_launch": int(flops)},
+            "D_range": [float(D.min()), float(D.max())],
+            "cpu_oracle_ms": {"raster": round(t_mask * 1e3, 1),
+                              "fft_blur_and_rescale": round(t_fft * 1e3, 1), "cores": 1}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -517,6 +592,8 @@ def main():
                     help="skip the autotuner (the size-based default kernel runs)")
     ap.add_argument("--sequence-steps", type=int, default=20_000,
                     help="Euler steps of the dataset-generator leg (0 skips it)")
+    ap.add_argument("--scar-size", type=int, default=1200,
+                    help="image side of the random D-map generator leg (0 skips it)")
     ap.add_argument("--slab-steps", type=int, default=4000,
                     help="Euler steps of the one-slab-of-eight leg (0 skips it)")
     args = ap.parse_args()
@@ -646,6 +723,11 @@ def main():
         log("dataset generator ...")
         extra["sequence"] = bench_sequence(args.seed, args.sequence_steps)
         log("sequence: %s" % json.dumps(extra["sequence"]))
+
+    if args.scar_size > 0:
+        log("scar D-map generator ...")
+        extra["scar"] = bench_scar(args.seed, args.scar_size)
+        log("scar: %s" % json.dumps(extra["scar"]))
 
     cpu = None
     if not args.no_cpu:

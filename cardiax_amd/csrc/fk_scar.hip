@@ -65,28 +65,78 @@ k_raster_polygons(float* __restrict__ mask, int H, int W, const double* __restri
   mask[(size_t)r * W + c] = on ? 1.0f : 0.0f;
 }
 
-// One output per thread; the kernel is read through the scalar cache (every
-// lane of a wave reads the same tap).
+// R outputs down a column per thread. With d = s - (i0 + oi) the offset of
+// input row s from the thread's first output row, output r takes tap row
+// p = r - d. The thread walks d over [-(kh-1), R-1] in chunks of R rows and,
+// within a chunk, every tap column q: R input values (one load each, reused
+// by up to R outputs; the chunk's rows stay in L1 across q) and the chunk's
+// 2R - 1 taps of column q, contiguous and wave-uniform in the padded,
+// transposed copy of the kernel (k_pad_kernel: zeros outside [0, kh), so no
+// test per tap). Out-of-image inputs are 0; input loads are clamped into the
+// image and the value selected away. Per output the sum runs over chunks,
+// then q, then d, in double; rounded once.
+constexpr int kConvRows = 8;
+constexpr int kConvPadFront = 2 * kConvRows - 2;
+
+__host__ __device__ constexpr int conv_padded_rows(int kh) { return kh + 3 * kConvRows - 3; }
+
 __global__ void __launch_bounds__(256)
-k_convolve_same(const float* __restrict__ src, int H, int W, const double* __restrict__ k,
+k_pad_kernel(const double* __restrict__ k, int kh, int kw, double* __restrict__ kt) {
+  const int kp = conv_padded_rows(kh);
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= kw * kp) return;
+  const int q = idx / kp, p = idx % kp - kConvPadFront;
+  kt[idx] = (p >= 0 && p < kh) ? k[(size_t)p * kw + q] : 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+k_convolve_same(const float* __restrict__ src, int H, int W, const double* __restrict__ kt,
                 int kh, int kw, float* __restrict__ dst) {
+  constexpr int R = kConvRows;
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (int)(threadIdx.x >> 6));
-  if (i >= H) return;
+  const int i0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * R);
+  if (i0 >= H) return;  // wave-uniform
   const int oi = (kh - 1) / 2, oj = (kw - 1) / 2;
-  double acc = 0.0;
-  for (int p = 0; p < kh; ++p) {
-    const int si = i + oi - p;
-    if (si < 0 || si >= H) continue;  // wave-uniform
-    const float* row = src + (size_t)si * W;
-    const double* krow = k + (size_t)p * kw;
+  const int kp = conv_padded_rows(kh);
+  const int n_chunks = (kh + 2 * R - 2) / R;
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int ch = 0; ch < n_chunks; ++ch) {
+    const int d0 = -(kh - 1) + ch * R;
+    const float* rows[R];
+    bool row_in[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int si = i0 + oi + d0 + m;  // wave-uniform
+      row_in[m] = si >= 0 && si < H;
+      rows[m] = src + (size_t)min(max(si, 0), H - 1) * W;
+    }
+    // taps p = -d0 - (R-1) + e of column q sit at kt[q kp + R - 1 - d0 + e]
+    const double* tcol = kt + (R - 1 - d0);
     for (int q = 0; q < kw; ++q) {
-      const int sj = j + oj - q;
-      const double x = (sj >= 0 && sj < W && j < W) ? (double)row[sj] : 0.0;
-      acc = fma(krow[q], x, acc);
+      const int c = j + oj - q;
+      const bool c_in = c >= 0 && c < W;
+      const int cc = min(max(c, 0), W - 1);
+      double x[R];
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const float v = rows[m][cc];
+        x[m] = (row_in[m] && c_in) ? (double)v : 0.0;
+      }
+      const double* t = tcol + (size_t)q * kp;
+#pragma unroll
+      for (int m = 0; m < R; ++m)    // d = d0 + m
+#pragma unroll
+        for (int r = 0; r < R; ++r)  // p = r - d: tap index r - m + R - 1
+          acc[r] = fma(t[r - m + R - 1], x[m], acc[r]);
     }
   }
-  if (j < W) dst[(size_t)i * W + j] = (float)acc;
+  if (j < W) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (i0 + r < H) dst[(size_t)(i0 + r) * W + j] = (float)acc[r];
+  }
 }
 
 // The scar-to-diffusivity map of random_diffusivity (deepx/generate.py:79-83,
@@ -230,15 +280,31 @@ size_t fk_rasterize_workspace_bytes(int32_t n_polys, int32_t n_vertices) {
          (size_t)n_polys * sizeof(PolyBox);
 }
 
+size_t fk_convolve_workspace_bytes(int32_t kh, int32_t kw) {
+  if (kh < 1 || kw < 1) return 0;
+  return (size_t)kw * conv_padded_rows(kh) * sizeof(double);
+}
+
 int32_t fk_convolve_same(const float* src, int32_t H, int32_t W, const double* kernel,
-                         int32_t kh, int32_t kw, float* dst, void* stream) {
+                         int32_t kh, int32_t kw, float* dst, void* work, size_t work_bytes,
+                         void* stream) {
   ErrorScope es;
-  if (!check_ptrs({src, kernel, dst})) return last_code();
+  if (!check_ptrs({src, kernel, dst, work})) return last_code();
   if (H < 1 || W < 1 || kh < 1 || kw < 1)
     return set_error(FK_ERR_ARG, "fk_convolve_same: empty image or kernel");
+  if (kh > (1 << 20) || kw > (1 << 20))
+    return set_error(FK_ERR_ARG, "fk_convolve_same: kernel of %d x %d is too large", kh, kw);
   if (src == dst) return set_error(FK_ERR_ARG, "fk_convolve_same: dst must not be src");
-  const dim3 grid((W + 63) / 64, (H + 3) / 4);
-  k_convolve_same<<<grid, 256, 0, (hipStream_t)stream>>>(src, H, W, kernel, kh, kw, dst);
+  const size_t need = fk_convolve_workspace_bytes(kh, kw);
+  if (work_bytes < need)
+    return set_error(FK_ERR_ARG, "workspace too small: need %zu bytes, got %zu", need,
+                     work_bytes);
+  hipStream_t s = (hipStream_t)stream;
+  double* kt = (double*)work;
+  const int n_pad = kw * conv_padded_rows(kh);
+  k_pad_kernel<<<(n_pad + 255) / 256, 256, 0, s>>>(kernel, kh, kw, kt);
+  const dim3 grid((W + 63) / 64, (H + 4 * kConvRows - 1) / (4 * kConvRows));
+  k_convolve_same<<<grid, 256, 0, s>>>(src, H, W, kt, kh, kw, dst);
   HIP_CHECK(hipGetLastError());
   return FK_OK;
 }
