@@ -268,8 +268,9 @@ def bench_adjoint(seed, size=256, n_steps=2000, frame_every=50, reps=2):
             "fwd_adjoint_hbm": {"bytes_per_cell_step": ADJ_FWD_BWD_BYTES,
                                 "achieved": round(fwd_adj, 1),
                                 "frac": round(fwd_adj / HBM_PEAK_GBS, 4)},
-            "roofline": leg_roofline(ADJ_BWD_BYTES * size * size, per_step,
-                                     "k_adj_main<true>", "%dx%d" % shape),
+            "roofline": dict(leg_roofline(ADJ_BWD_BYTES * size * size, per_step,
+                                          "k_adj_main<true>", "%dx%d" % shape),
+                             design_bytes_per_launch=ADJ_BWD_DESIGN_BYTES * size * size),
             "grad_finite": bool(torch.isfinite(D.grad).all().item())}
 
 
@@ -279,6 +280,10 @@ def bench_adjoint(seed, size=256, n_steps=2000, frame_every=50, reps=2):
 # and its recompute from the checkpoints (28).
 ADJ_BWD_BYTES = 48
 ADJ_FWD_BWD_BYTES = 104
+# What the fused reverse kernel moves by construction per cell: it also reads
+# the derived fields of its step (28), lam_v/lam_w, v, w, u, D, dL/dD and the
+# previous u (32 more in all) and writes the next step's derived fields (28).
+ADJ_BWD_DESIGN_BYTES = 104
 
 
 def adjoint_step_seconds(st, prm, D, stims, shape, t_mid, n_short=10, n_long=40):
@@ -590,6 +595,9 @@ def main():
                     help="Adam iterations of the batched inverse problem (0 = skip it)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (the size-based default kernel runs)")
+    ap.add_argument("--variant", default=None,
+                    help="run the single-tissue solve with this step kernel (no tuning), "
+                         "e.g. stream2 or streamt8; the other legs keep their own choice")
     ap.add_argument("--sequence-steps", type=int, default=20_000,
                     help="Euler steps of the dataset-generator leg (0 skips it)")
     ap.add_argument("--scar-size", type=int, default=1200,
@@ -633,7 +641,14 @@ def main():
     from cardiax_amd import autotune
 
     tuned = None
-    if not args.no_tune:  # setup, outside the timed region
+    forced = None
+    if args.variant is not None:
+        from cardiax_amd import _lib
+
+        if args.variant not in _lib.VARIANTS:
+            sys.exit("unknown variant %r (one of %s)" % (args.variant, sorted(_lib.VARIANTS)))
+        forced = _lib.lib().fk_set_forward_variant(_lib.VARIANTS[args.variant])
+    elif not args.no_tune:  # setup, outside the timed region
         # on the workload's own state after S2 (the spiral regime that fills
         # most of the run), with its parameters, D-map and stimuli
         t_mid = min(sim.t1 - 300, sim.meta["s2_start"] + (sim.t1 - sim.meta["s2_start"]) // 4)
@@ -694,6 +709,9 @@ def main():
         plain = H * W * n / (e0.elapsed_time(e1) / 1e3) / 1e9
         extra["plain_fp32_kernel_gcups"] = round(plain, 3)
         extra["speedup_vs_plain_fp32"] = round(extra["event_gcups"] / plain, 3)
+
+    if forced is not None:  # the other legs run their own kernels again
+        _lib.lib().fk_set_forward_variant(forced)
 
     if args.batch_sims > 0:
         log("batched workload ...")
