@@ -63,6 +63,14 @@ class FkSim(ctypes.Structure):
                 ("dx", c_f)]
 
 
+class FkRolloutSpec(ctypes.Structure):
+    """struct fk_rollout_spec: one problem of a batched rollout."""
+    _fields_ = [("params", FkParams), ("D", c_vp), ("v0", c_vp), ("w0", c_vp), ("u0", c_vp),
+                ("H", c_i32), ("W", c_i32), ("n_stimuli", c_i32),
+                ("stimuli", FkStimulus * FK_BATCH_MAX_STIMULI), ("t0", c_d), ("dt", c_f),
+                ("dx", c_f), ("frames", c_vp), ("g_frames", c_vp), ("grad_D", c_vp)]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "fk_last_error": (ctypes.c_char_p, []),
@@ -130,6 +138,12 @@ _SIGNATURES = {
     "fk_convolve_workspace_bytes": (c_sz, [c_i32, c_i32]),
     "fk_scar_to_diffusivity": (c_i32, [c_vp, c_i64, c_d, c_d, c_vp, c_vp, c_sz, c_vp]),
     "fk_batch_create": (c_i32, [ctypes.POINTER(FkSim), c_i32, ctypes.POINTER(c_vp)]),
+    "fk_rollout_batch_create": (c_i32, [ctypes.POINTER(FkRolloutSpec), c_i32, c_i32, c_i32,
+                                        c_i32, ctypes.POINTER(c_vp)]),
+    "fk_rollout_batch_device_bytes": (c_sz, [c_vp]),
+    "fk_rollout_batch_forward": (c_i32, [c_vp, c_vp]),
+    "fk_rollout_batch_backward": (c_i32, [c_vp, c_vp, c_vp]),
+    "fk_rollout_batch_destroy": (c_i32, [c_vp]),
     "fk_batch_run": (c_i32, [c_vp, c_i64, c_i32, c_vp]),
     "fk_batch_autotune": (c_i32, [c_vp, c_i32, c_vp, ctypes.POINTER(c_f),
                                   ctypes.POINTER(c_i32)]),

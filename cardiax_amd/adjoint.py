@@ -154,3 +154,147 @@ def rollout(D: torch.Tensor, tau_d, tau_r, tau_si, state0, params, stimuli: Sequ
            "dt": float(dt), "dx": float(dx), "checkpoint_every": checkpoint_every,
            "tau_host": host}
     return _Rollout.apply(D, taus[0], taus[1], taus[2], cfg)
+
+
+def checkpoint_interval(n_steps: int, checkpoint_every: int | None = None) -> int:
+    """Steps between checkpoints: ``checkpoint_every`` or ceil(sqrt(n_steps))."""
+    return int(checkpoint_every) if checkpoint_every else max(1, int(math.ceil(math.sqrt(n_steps))))
+
+
+class RolloutBatch:
+    """Many ``rollout`` problems planned together (fk_rollout_batch_*): the
+    forward solves and the reverse passes of all of them run as one launch
+    per Euler step and one per reverse step, over work lists that hold every
+    tile of every grid. Each problem is ``rollout(D, tau_d, tau_r, tau_si,
+    state0, params, stimuli, t0, n_steps, frame_every, dt, dx)`` with the taus
+    of ``params`` held constant; the results (frames and dL/dD) are bitwise
+    those of ``rollout`` per problem with the same checkpoint interval.
+
+    ``problems``: (state0, params, stimuli, t0) per problem; the grids may
+    differ in size. The plan owns the device buffers: ``D`` (one tensor per
+    problem, overwritten by ``forward``), ``frames``, ``g_frames`` and
+    ``grad_D``. Use it as a context manager or call ``close``."""
+
+    def __init__(self, problems, n_steps: int, frame_every: int, dt: float, dx: float,
+                 checkpoint_every: int | None = None):
+        dev = require_gpu("RolloutBatch")
+        n_steps, fe = int(n_steps), int(frame_every)
+        if n_steps < fe or fe < 1:
+            raise ValueError("need n_steps >= frame_every >= 1 (got %d, %d)" % (n_steps, fe))
+        self.n_steps, self.frame_every = n_steps, fe
+        self.S = checkpoint_interval(n_steps, checkpoint_every)
+        n_frames = n_steps // fe
+        self.D, self.frames, self.g_frames, self.grad_D, self.g_tau = [], [], [], [], None
+        self._keep = []
+        specs = (_lib.FkRolloutSpec * len(problems))()
+        for i, (state0, params, stimuli, t0) in enumerate(problems):
+            st = State(*(on_device(x, dev).contiguous() for x in state0))
+            H, W = st.u.shape
+            stim = _Stimuli(stimuli, dev, (H, W))
+            if stim.n > _lib.FK_BATCH_MAX_STIMULI:
+                raise NotImplementedError("at most %d stimuli per problem, got %d"
+                                          % (_lib.FK_BATCH_MAX_STIMULI, stim.n))
+            D = torch.empty((H, W), dtype=torch.float32, device=dev)
+            fr = torch.empty((n_frames, H, W), dtype=torch.float32, device=dev)
+            gf = torch.zeros((n_frames, H, W), dtype=torch.float32, device=dev)
+            gD = torch.empty((H, W), dtype=torch.float32, device=dev)
+            self.D.append(D)
+            self.frames.append(fr)
+            self.g_frames.append(gf)
+            self.grad_D.append(gD)
+            self._keep.append((st, stim))
+            sp = specs[i]
+            sp.params = _lib.make_params(params)
+            sp.D, sp.v0, sp.w0, sp.u0 = D.data_ptr(), st.v.data_ptr(), st.w.data_ptr(), \
+                st.u.data_ptr()
+            sp.H, sp.W, sp.n_stimuli = H, W, stim.n
+            for k in range(stim.n):
+                sp.stimuli[k] = stim.arr[k]
+            sp.t0, sp.dt, sp.dx = float(_scalar(t0)), float(dt), float(dx)
+            sp.frames, sp.g_frames, sp.grad_D = fr.data_ptr(), gf.data_ptr(), gD.data_ptr()
+        torch.cuda.synchronize()  # initial states made on other streams are complete
+        self._h = ctypes.c_void_p()
+        _lib.check(_lib.lib().fk_rollout_batch_create(specs, len(problems), n_steps, fe, self.S,
+                                                      ctypes.byref(self._h)))
+        self.n = len(problems)
+
+    @property
+    def device_bytes(self) -> int:
+        """Device memory of the plan itself (checkpoints, segment states,
+        cotangents, workspaces), bytes."""
+        return int(_lib.lib().fk_rollout_batch_device_bytes(self._h))
+
+    def forward(self, Ds):
+        """Frames of every problem at the D-maps ``Ds`` (copied into the plan's
+        D buffers): the plan's ``frames`` tensors, overwritten."""
+        self._require_open()
+        with torch.no_grad():
+            for dst, src in zip(self.D, Ds):
+                dst.copy_(src)
+        _lib.check(_lib.lib().fk_rollout_batch_forward(self._h, stream_handle()))
+        return self.frames
+
+    def backward(self, g_frames=None, tau_grads: bool = False):
+        """dLoss/dD of every problem from the frame cotangents (the plan's
+        ``g_frames`` when None) of the last ``forward``. Returns the plan's
+        ``grad_D`` tensors and, with ``tau_grads``, an (n, 3) tensor of
+        gradients with respect to (tau_d, tau_r, tau_si)."""
+        self._require_open()
+        if g_frames is not None:
+            with torch.no_grad():
+                for dst, src in zip(self.g_frames, g_frames):
+                    dst.copy_(src)
+        gt = None
+        if tau_grads:
+            gt = torch.empty((self.n, 3), dtype=torch.float32, device=self.grad_D[0].device)
+        _lib.check(_lib.lib().fk_rollout_batch_backward(
+            self._h, gt.data_ptr() if gt is not None else None, stream_handle()))
+        return (self.grad_D, gt) if tau_grads else self.grad_D
+
+    def _require_open(self):
+        if self._h is None or not self._h.value:
+            raise RuntimeError("the rollout batch is closed")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.check(_lib.lib().fk_rollout_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+
+class _RolloutBatchFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, plan, *Ds):
+        ctx.plan = plan
+        frames = plan.forward(Ds)
+        return tuple(f.clone() for f in frames)
+
+    @staticmethod
+    def backward(ctx, *g_frames):
+        plan = ctx.plan
+        zeros = [torch.zeros_like(f) if g is None else g for g, f in zip(g_frames, plan.frames)]
+        grads = plan.backward([g.contiguous().to(torch.float32) for g in zeros])
+        return (None,) + tuple(g.clone() for g in grads)
+
+
+def rollout_batch(plan: RolloutBatch, Ds):
+    """The frames of every problem of ``plan`` at ``Ds``, differentiable with
+    respect to each D (the batched ``rollout``)."""
+    if len(Ds) != plan.n:
+        raise ValueError("%d D-maps for %d problems" % (len(Ds), plan.n))
+    for D, ref in zip(Ds, plan.D):
+        if tuple(D.shape) != tuple(ref.shape):
+            raise ValueError("D has shape %s, the problem's grid is %s"
+                             % (tuple(D.shape), tuple(ref.shape)))
+    return _RolloutBatchFn.apply(plan, *Ds)

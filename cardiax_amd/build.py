@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcardiax_amd.so")
 
 SOURCES = ["fk_common.hip", "fk_forward.hip", "fk_batch.hip", "fk_adjoint.hip", "fk_io.hip",
-           "fk_stimulus.hip", "fk_noalloc.hip", "fk_scar.hip"]
+           "fk_stimulus.hip", "fk_noalloc.hip", "fk_scar.hip", "fk_rollout_batch.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -39,7 +39,8 @@ HIPCC_FLAGS = [
 # `#pragma unroll`; a loop left rolled would index its register windows at run
 # time and move them to scratch memory.
 FILE_FLAGS = {"fk_forward.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"],
-              "fk_batch.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"]}
+              "fk_batch.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"],
+              "fk_rollout_batch.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"]}
 
 # The host pass does not know the packed-fp32 feature and says so.
 _NOISE = "'-packed-fp32-ops' is not a recognized feature"

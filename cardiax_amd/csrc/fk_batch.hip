@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "cardiax_amd.h"
+#include "fk_batch.h"
 #include "fk_device.h"
 #include "fk_host.h"
 #include "fk_stream.h"
@@ -44,11 +45,6 @@ constexpr int kGraphSteps = 32;   // Euler steps recorded in one graph (even)
 // a two-step option).
 constexpr int one_step_opt(int o) { return o == 4 ? 2 : o == 5 ? 3 : o; }
 
-struct BatchStim {
-  double start, duration, period;
-  const float* field;
-};
-
 struct SimDesc {
   Coef c;
   const float* D;
@@ -63,13 +59,6 @@ struct SimDesc {
 struct TileItem {
   int sim, cs, rs, flags;  // flags: bit0 band near a row edge, bit1 strip near a column edge
 };
-
-__device__ __forceinline__ bool active_dev(double t, const BatchStim& s) {
-  if (!(t >= s.start)) return false;
-  double r = fmod(s.start - t + 1.0, s.period);
-  if (r != 0.0 && ((r < 0.0) != (s.period < 0.0))) r += s.period;
-  return r < s.duration;
-}
 
 // Stimulus activity of every simulation at the n_rel steps from the device
 // step counter on: bit k of mask[r * n + i] = stimulus k of simulation i is
@@ -87,29 +76,6 @@ __global__ void k_batch_masks(const SimDesc* __restrict__ sims, int n,
     if (active_dev(t, S.stim[k])) m |= 1u << k;
   mask[idx] = m;
 }
-
-// Wave-uniform copies (scalar registers) of descriptor values.
-__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ float uni(float x) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
-}
-template <class P>
-__device__ __forceinline__ P* uni(P* p) {
-  const uint64_t a = (uint64_t)p;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-  return (P*)(((uint64_t)hi << 32) | lo);
-}
-
-// The simulation's stimuli with this step's activity mask.
-struct BatchStimView {
-  const BatchStim* s;
-  int n;
-  uint32_t mask;
-  __device__ int count() const { return n; }
-  __device__ bool on(int m) const { return (mask >> m) & 1u; }
-  __device__ const float* field(int m) const { return s[m].field; }
-};
 
 // One Euler step of every tile (bands of R rows). Reads state[parity],
 // writes state[parity ^ 1].

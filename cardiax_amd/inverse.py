@@ -14,9 +14,13 @@ with its non-conservative D_x u_x term does not damp. The interpolation is
 two small matrix products (deterministic, unlike scattered atomics).
 
 The reference has no such solver loop (its deepx package learns the inverse map
-with a network instead). The instances run interleaved on several HIP streams
-so that small grids, which leave most of the GPU idle, overlap; no step of
-the loop synchronises the host with the device.
+with a network instead). By default all instances advance together through
+one ``adjoint.RolloutBatch`` plan: each Euler step of the forward solves, and
+each reverse step of the adjoint, is one launch over a work list holding every
+tile of every instance, with results bitwise equal to the per-instance
+rollouts. ``batched=False`` runs the per-instance rollouts instead,
+interleaved on several HIP streams. No step of the loop synchronises the host
+with the device.
 """
 from __future__ import annotations
 
@@ -94,21 +98,38 @@ class _Param(NamedTuple):
         return self.Ry @ self.theta @ self.Rx.t()
 
 
+def _plan(instances: Sequence[Instance], dt: float, dx: float) -> adjoint.RolloutBatch:
+    n_steps, fe = instances[0].n_steps, instances[0].frame_every
+    if any(x.n_steps != n_steps or x.frame_every != fe for x in instances):
+        raise ValueError("a batched plan needs one n_steps and frame_every for all instances")
+    return adjoint.RolloutBatch([(x.state0, x.params, x.stimuli, 0) for x in instances],
+                                n_steps, fe, dt, dx)
+
+
+def _frame_losses(frames, instances) -> List[torch.Tensor]:
+    """The frame MSE of each instance (accumulated in float64)."""
+    out = []
+    for fr, inst in zip(frames, instances):
+        diff = (fr - inst.target).double()
+        out.append((diff * diff).mean())
+    return out
+
+
 def recover(instances: Sequence[Instance], iters: int = 50, lr: float = 2e-5,
             n_streams: int = 8, coarse: int = 8, dt: float = wl.DT,
-            dx: float = wl.DX) -> dict:
+            dx: float = wl.DX, batched: bool = True) -> dict:
     """Runs ``iters`` Adam iterations of forward + adjoint on every instance.
     Returns the estimates and per-instance losses and D errors (relative L2
-    distance to the hidden map, before and after), plus the wall time."""
+    distance to the hidden map, before and after), plus the wall time.
+    ``batched``: all instances through one RolloutBatch plan (they must share
+    n_steps and frame_every); otherwise per-instance rollouts on
+    ``n_streams`` streams."""
     dev = require_gpu("inverse")
+    if batched:
+        return _recover_batched(instances, iters, lr, coarse, dt, dx, dev)
     streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
-    params, opts = [], []
-    for inst in instances:
-        H, W = inst.D_true.shape
-        h, w = max(2, -(-H // coarse) + 1), max(2, -(-W // coarse) + 1)
-        th = torch.full((h, w), D_MAX, dtype=torch.float32, device=dev, requires_grad=True)
-        params.append(_Param(th, interp_matrix(H, h, dev), interp_matrix(W, w, dev)))
-        opts.append(torch.optim.Adam([th], lr=lr))
+    params = _init_params(instances, coarse, dev)
+    opts = [torch.optim.Adam([prm.theta], lr=lr) for prm in params]
     losses = torch.zeros((len(instances), iters), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -124,6 +145,44 @@ def recover(instances: Sequence[Instance], iters: int = 50, lr: float = 2e-5,
                 losses[k, it] = loss.detach()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    return _result(instances, params, losses, wall)
+
+
+def _init_params(instances, coarse, dev):
+    params, opts = [], []
+    for inst in instances:
+        H, W = inst.D_true.shape
+        h, w = max(2, -(-H // coarse) + 1), max(2, -(-W // coarse) + 1)
+        th = torch.full((h, w), D_MAX, dtype=torch.float32, device=dev, requires_grad=True)
+        params.append(_Param(th, interp_matrix(H, h, dev), interp_matrix(W, w, dev)))
+    return params
+
+
+def _recover_batched(instances, iters, lr, coarse, dt, dx, dev) -> dict:
+    params = _init_params(instances, coarse, dev)
+    opts = [torch.optim.Adam([prm.theta], lr=lr) for prm in params]
+    losses = torch.zeros((len(instances), iters), dtype=torch.float64, device=dev)
+    with _plan(instances, dt, dx) as plan:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for it in range(iters):
+            for opt in opts:
+                opt.zero_grad(set_to_none=True)
+            Ds = [prm.D().contiguous() for prm in params]
+            frames = adjoint.rollout_batch(plan, Ds)
+            ls = _frame_losses(frames, instances)
+            torch.stack(ls).sum().backward()
+            for k, (opt, prm) in enumerate(zip(opts, params)):
+                opt.step()
+                with torch.no_grad():  # interpolation keeps D within the nodes' range
+                    prm.theta.clamp_(D_MIN, D_MAX)
+                losses[k, it] = ls[k].detach()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    return _result(instances, params, losses, wall)
+
+
+def _result(instances, params, losses, wall) -> dict:
     err0, err1, Ds = [], [], []
     for inst, prm in zip(instances, params):
         with torch.no_grad():
@@ -151,11 +210,25 @@ def _loss_backward(inst: Instance, D: torch.Tensor, dt: float, dx: float) -> tor
 
 
 def d_gradients(instances: Sequence[Instance], Ds: Sequence[torch.Tensor], n_streams: int = 8,
-                dt: float = wl.DT, dx: float = wl.DX) -> List[torch.Tensor]:
+                dt: float = wl.DT, dx: float = wl.DX, batched: bool = False,
+                plan: adjoint.RolloutBatch | None = None) -> List[torch.Tensor]:
     """The gradient of each instance's frame MSE with respect to its full D
-    field at Ds[k], by the pass ``recover`` makes in every iteration
-    (instances interleaved on ``n_streams`` HIP streams)."""
+    field at Ds[k], by the pass ``recover`` makes in every iteration: through
+    one RolloutBatch plan (``batched``; ``plan`` may be given to reuse one),
+    or per instance, interleaved on ``n_streams`` HIP streams."""
     require_gpu("inverse")
+    if batched or plan is not None:
+        own = plan is None
+        plan = _plan(instances, dt, dx) if own else plan
+        try:
+            Dk = [D.detach().clone().requires_grad_(True) for D in Ds]
+            frames = adjoint.rollout_batch(plan, Dk)
+            torch.stack(_frame_losses(frames, instances)).sum().backward()
+            torch.cuda.synchronize()
+            return [D.grad for D in Dk]
+        finally:
+            if own:
+                plan.close()
     streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
     torch.cuda.synchronize()  # inputs made on other streams are complete
     out = []

@@ -375,6 +375,57 @@ int32_t fk_batch_n_items(const fk_batch* b);
 int32_t fk_batch_destroy(fk_batch* b);
 
 /* ---------------------------------------------------------------------------
+ * Batched rollout with its adjoint: many frame-loss problems (the forward
+ * solve of cardiax/solve.py:92-100 sampled every frame_every steps, and the
+ * reverse pass of fk_adjoint_segment) advanced together, one launch per
+ * Euler step and one per reverse step for all of them. No reference
+ * counterpart: it batches what the per-grid calls fk_forward_trajectory,
+ * fk_adjoint_prepare, fk_adjoint_segment and fk_adjoint_tau_grads do for one
+ * problem, and its results are bitwise those of that per-grid sequence with
+ * the same checkpoint interval.
+ * ------------------------------------------------------------------------- */
+
+typedef struct fk_rollout_spec {
+  fk_params params;
+  const float* D;   /* H x W, device; read by every forward and backward call */
+  const float* v0;  /* initial state, device */
+  const float* w0;
+  const float* u0;
+  int32_t H, W;
+  int32_t n_stimuli;
+  fk_stimulus stimuli[FK_BATCH_MAX_STIMULI];
+  double t0;        /* time index of the first step */
+  float dt, dx;
+  float* frames;         /* n_steps / frame_every planes of H x W (forward output) */
+  const float* g_frames; /* the same shape: dLoss/dframes (backward input) */
+  float* grad_D;         /* H x W: dLoss/dD (backward output) */
+} fk_rollout_spec;
+
+typedef struct fk_rollout_batch fk_rollout_batch; /* opaque */
+
+/* Plans n problems of n_steps Euler steps each, frames every frame_every
+ * steps (frame j-1 = u after j * frame_every steps), checkpoints every
+ * checkpoint_every steps. Allocates (once) the checkpoints, one segment of
+ * states per problem, the cotangents and the adjoint workspaces; the
+ * caller's buffers named in the specs must stay valid until destroy.
+ * Synchronous. */
+int32_t fk_rollout_batch_create(const fk_rollout_spec* specs, int32_t n, int32_t n_steps,
+                                int32_t frame_every, int32_t checkpoint_every,
+                                fk_rollout_batch** out);
+/* Device memory the plan holds, bytes. */
+size_t fk_rollout_batch_device_bytes(const fk_rollout_batch* b);
+/* The forward rollout of every problem from its D and initial state: writes
+ * the frames and keeps the checkpoints. Stream-ordered. */
+int32_t fk_rollout_batch_forward(fk_rollout_batch* b, void* stream);
+/* The reverse pass from g_frames after a forward call with the same D:
+ * writes grad_D and, when g_tau is not null, the gradients with respect to
+ * (tau_d, tau_r, tau_si) of every problem (3 n floats, device). Stream-
+ * ordered. */
+int32_t fk_rollout_batch_backward(fk_rollout_batch* b, float* g_tau, void* stream);
+/* Waits for the device and frees the plan. */
+int32_t fk_rollout_batch_destroy(fk_rollout_batch* b);
+
+/* ---------------------------------------------------------------------------
  * Checkpoint output of the dataset generator (deepx/generate.py:135-210).
  * ------------------------------------------------------------------------- */
 
