@@ -336,32 +336,56 @@ def adjoint_step_seconds(st, prm, D, stims, shape, t_mid, n_short=10, n_long=40)
     return (t_long - t_short) / (n_long - n_short)
 
 
-def bench_inverse(seed, n=32, iters=50, n_steps=1000, frame_every=50):
+def bench_inverse(seed, n=32, iters=100, n_steps=1000, frame_every=50, per_instance_iters=2):
     """Batched inverse problem: n instances of 128^2..512^2 each recover a
-    hidden scar D-map by `iters` Adam iterations of forward + adjoint."""
+    hidden scar D-map by `iters` Adam iterations of forward + adjoint, all
+    instances through one batched rollout plan (one launch per forward step
+    and per reverse step for all of them). For comparison, a few iterations
+    of the per-instance path (one rollout per instance, 8 streams), which
+    gives the same bits."""
     import numpy as np
+    import torch
 
     from cardiax_amd import inverse
+    from cardiax_amd import workloads as wl
 
     inst = inverse.make_instances(seed, n=n, n_steps=n_steps, frame_every=frame_every)
     inverse.recover(inst[:2], iters=1)  # warm-up (kernels, allocator)
-    tf = inverse.forward_only_seconds(inst)
-    res = inverse.recover(inst, iters=iters)
     cells = sum(int(x.D_true.numel()) for x in inst)
+    with inverse._plan(inst, wl.DT, wl.DX) as plan:  # forward only
+        Ds = [torch.full_like(x.D_true, inverse.D_MAX) for x in inst]
+        plan.forward(Ds)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.forward(Ds)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t0
+        plan_gb = plan.device_bytes / 1e9
+    res = inverse.recover(inst, iters=iters)
+    per = inverse.recover(inst, iters=per_instance_iters, batched=False)
     loss = res["loss"]
+    d0, d1 = np.array(res["d_err_initial"]), np.array(res["d_err_final"])
     return {"workload": "%d instances, sides %s, %d Adam iterations of forward + adjoint, "
                         "%d Euler steps each, frames every %d, float64 loss"
                         % (n, sorted({int(x.D_true.shape[0]) for x in inst}), iters, n_steps,
                            frame_every),
             "seconds": round(res["seconds"], 3),
             "fwd_adjoint_gcups": round(cells * n_steps * iters / res["seconds"] / 1e9, 3),
+            "per_instance_fwd_adjoint_gcups": round(
+                cells * n_steps * per_instance_iters / per["seconds"] / 1e9, 3),
+            "forward_only_seconds": round(tf, 4),
+            "forward_only_gcups": round(cells * n_steps / tf / 1e9, 3),
             "cost_ratio": round(res["seconds"] / iters / tf, 2),
+            "plan_device_gb": round(plan_gb, 3),
             # summed over instances (a few see no wave reach their scars, so their
             # loss starts near 1e-17 and a per-instance ratio says nothing)
-            "loss_sum_final_over_initial": round(float(loss[:, -1].sum() / loss[:, 0].sum()), 4),
-            "loss_ratio_median": round(float(np.median(loss[:, -1] / loss[:, 0])), 4),
-            "d_err_initial": round(float(np.mean(res["d_err_initial"])), 4),
-            "d_err_final": round(float(np.mean(res["d_err_final"])), 4)}
+            "loss_sum_final_over_initial": round(float(loss[:, -1].sum() / loss[:, 0].sum()), 6),
+            "loss_ratio_median": round(float(np.median(loss[:, -1] / loss[:, 0])), 6),
+            "d_err_initial": round(float(np.mean(d0)), 4),
+            "d_err_final": round(float(np.mean(d1)), 4),
+            "d_err_per_instance": [[int(x.D_true.shape[0]), round(float(a), 4), round(float(b), 4)]
+                                   for x, a, b in zip(inst, d0, d1)],
+            "instances_improved": int((d1 < d0).sum())}
 
 
 def bench_heun(seed, size=1024, n_steps=2000):
@@ -591,7 +615,7 @@ def main():
                     help="simulations of the batched workload (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--inverse-iters", type=int, default=50,
+    ap.add_argument("--inverse-iters", type=int, default=100,
                     help="Adam iterations of the batched inverse problem (0 = skip it)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (the size-based default kernel runs)")

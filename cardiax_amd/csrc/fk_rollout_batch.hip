@@ -85,6 +85,7 @@ k_roll_step(const RollDesc* __restrict__ sims, const RollTile* __restrict__ item
   const int it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (int)(threadIdx.x >> 6));
   if (it >= n_items) return;
   RollTile item = items[it];
+  if (uni(item.sim) < 0) return;  // padding of the XCD order
   item.sim = uni(item.sim);
   item.cs = uni(item.cs);
   item.rs = uni(item.rs);
@@ -174,6 +175,7 @@ k_roll_pre(const RollDesc* __restrict__ sims, const RollBlock* __restrict__ bloc
            int k, int j, int fe) {
   __shared__ AdjAxisTab tab[2];
   const RollBlock b = blocks[blockIdx.x];
+  if (b.sim < 0) return;  // padding of the XCD order (block-uniform)
   const RollDesc& S = sims[b.sim];
   adj_pre_tile(roll_state(S, seg, k, 2), S.D, S.Dx, S.Dy, S.lam + 2 * S.cells,
                roll_inject(S, j, fe), S.H, S.W, S.tab, S.pre[0], b.bx, b.by, tab);
@@ -190,6 +192,7 @@ k_roll_main(const RollDesc* __restrict__ sims, const RollBlock* __restrict__ blo
   __shared__ AdjAxisTab tab[2];
   __shared__ float4 xch[kAdjThreads / 2];
   const RollBlock b = blocks[blockIdx.x];
+  if (b.sim < 0) return;  // padding of the XCD order (block-uniform)
   const RollDesc& S = sims[b.sim];
   const size_t n = S.cells;
   float* lin = cur ? S.lam_alt : S.lam;
@@ -223,6 +226,36 @@ struct fk_rollout_batch {
   int n_tiles = 0, n_pre = 0, n_main = 0;
   size_t max_cells = 0;
 };
+
+// Work-list order for the chip's placement of workgroup w on XCD w % 8 (each
+// XCD has its own L2): `runs` are groups of consecutive workgroups that share
+// halo rows (a column of tiles); each run goes whole to the XCD with the least
+// work so far, and the XCD lists are interleaved, short ones padded with
+// `pad` items that exit at once. `per_wg` items make one workgroup.
+constexpr int kXcds = 8;
+
+template <class T>
+static std::vector<T> xcd_order(const std::vector<std::vector<T>>& runs, int per_wg, T pad) {
+  std::vector<std::vector<T>> lists(kXcds);
+  for (const auto& r : runs) {
+    size_t best = 0;
+    for (int x = 1; x < kXcds; ++x)
+      if (lists[x].size() < lists[best].size()) best = x;
+    lists[best].insert(lists[best].end(), r.begin(), r.end());
+  }
+  size_t m = 0;
+  for (auto& l : lists) {
+    while (l.size() % per_wg) l.push_back(pad);
+    m = std::max(m, l.size() / per_wg);
+  }
+  std::vector<T> out;
+  out.reserve(m * kXcds * per_wg);
+  for (size_t j = 0; j < m; ++j)
+    for (int x = 0; x < kXcds; ++x)
+      for (int k = 0; k < per_wg; ++k)
+        out.push_back(j * per_wg + k < lists[x].size() ? lists[x][j * per_wg + k] : pad);
+  return out;
+}
 
 static int32_t roll_check(const fk_rollout_spec& s, int i) {
   if (!check_params(&s.params) || !check_grid(s.H, s.W) ||
@@ -274,9 +307,11 @@ int32_t fk_rollout_batch_create(const fk_rollout_spec* specs, int32_t n, int32_t
   b->S = std::min(checkpoint_every, n_steps);
   b->n_seg = (n_steps + b->S - 1) / b->S;
   b->specs.assign(specs, specs + n);
-  // work lists
-  std::vector<RollTile> tiles;
-  std::vector<RollBlock> pre, main;
+  // work lists: runs of vertically adjacent tiles (which share halo rows),
+  // placed whole on one XCD each
+  constexpr int kRunTiles = 16, kRunPre = 8, kRunMain = 16;
+  std::vector<std::vector<RollTile>> tile_runs;
+  std::vector<std::vector<RollBlock>> pre_runs, main_runs;
   for (int i = 0; i < n; ++i) {
     const fk_rollout_spec& s = specs[i];
     const StreamGeom g = make_stream_geom<kRollRows>(s.H, s.W);
@@ -289,14 +324,24 @@ int32_t fk_rollout_batch_create(const fk_rollout_spec* specs, int32_t n, int32_t
         const bool xg = edge_band(t.rs, kRollRows, s.H, s.W);
         const bool yg = t.cs < 3 || t.cs + kStripOut + 3 > s.W;
         t.flags = (xg ? 1 : 0) | (yg ? 2 : 0);
-        tiles.push_back(t);
+        if (band % kRunTiles == 0) tile_runs.emplace_back();
+        tile_runs.back().push_back(t);
       }
     const int gx = (s.W + 63) / 64;
-    for (int by = 0; by < (s.H + 3) / 4; ++by)
-      for (int bx = 0; bx < gx; ++bx) pre.push_back(RollBlock{i, bx, by});
-    for (int by = 0; by < (s.H + 1) / 2; ++by)
-      for (int bx = 0; bx < gx; ++bx) main.push_back(RollBlock{i, bx, by});
+    for (int bx = 0; bx < gx; ++bx)
+      for (int by = 0; by < (s.H + 3) / 4; ++by) {
+        if (by % kRunPre == 0) pre_runs.emplace_back();
+        pre_runs.back().push_back(RollBlock{i, bx, by});
+      }
+    for (int bx = 0; bx < gx; ++bx)
+      for (int by = 0; by < (s.H + 1) / 2; ++by) {
+        if (by % kRunMain == 0) main_runs.emplace_back();
+        main_runs.back().push_back(RollBlock{i, bx, by});
+      }
   }
+  const std::vector<RollTile> tiles = xcd_order(tile_runs, 4, RollTile{-1, 0, 0, 0});
+  const std::vector<RollBlock> pre = xcd_order(pre_runs, 1, RollBlock{-1, 0, 0});
+  const std::vector<RollBlock> main = xcd_order(main_runs, 1, RollBlock{-1, 0, 0});
   b->n_tiles = (int)tiles.size();
   b->n_pre = (int)pre.size();
   b->n_main = (int)main.size();
