@@ -68,12 +68,29 @@ struct RollBlock {
   int sim, bx, by;
 };
 
+// A pointer field of a descriptor, read as a global-memory pointer. Pointers
+// read from memory are generic, and the tile bodies index them directly, so
+// without this they compile to flat loads (which also wait on the LDS
+// counter). The field's bits are loaded as an address-space-1 pointer and
+// then converted, which lets the compiler keep every access global. Every
+// field lives in device memory.
+template <class T>
+__device__ __forceinline__ T* gfield(T* const& f) {
+  typedef T __attribute__((address_space(1)))* GT;
+  GT g;
+  __builtin_memcpy(&g, &f, sizeof g);
+  return (T*)g;
+}
+__device__ __forceinline__ AdjPre gfield(const AdjPre& a) {
+  return AdjPre{gfield(a.Llap), gfield(a.QPx), gfield(a.QPy), gfield(a.Q2)};
+}
+
 // State x_{c + k} of segment `seg` (c = seg S): the checkpoint at k = 0, else
 // slot k - 1 of the segment's trajectory. Plane f: 0 v, 1 w, 2 u.
 __device__ __forceinline__ const float* roll_state(const RollDesc& S, int seg, int k, int f) {
   if (k == 0)
-    return seg == 0 ? S.x0[f] : S.ckpt + ((size_t)(seg - 1) * 3 + f) * S.cells;
-  return S.traj + ((size_t)(k - 1) * 3 + f) * S.cells;
+    return seg == 0 ? gfield(S.x0[f]) : gfield(S.ckpt) + ((size_t)(seg - 1) * 3 + f) * S.cells;
+  return gfield(S.traj) + ((size_t)(k - 1) * 3 + f) * S.cells;
 }
 
 // One Euler step of every tile: x_{c+k} -> trajectory slot k. masks: this
@@ -165,7 +182,7 @@ __global__ void __launch_bounds__(256) k_roll_lam_back(const RollDesc* __restric
 
 // Frame cotangent injected into x_j (null when j is not a frame step).
 __device__ __forceinline__ const float* roll_inject(const RollDesc& S, int j, int fe) {
-  return (j > 0 && j % fe == 0) ? S.g_frames + (size_t)(j / fe - 1) * S.cells : nullptr;
+  return (j > 0 && j % fe == 0) ? gfield(S.g_frames) + (size_t)(j / fe - 1) * S.cells : nullptr;
 }
 
 // Derived fields (set 0) of the last step k of segment `seg`, whose output
@@ -177,8 +194,9 @@ k_roll_pre(const RollDesc* __restrict__ sims, const RollBlock* __restrict__ bloc
   const RollBlock b = blocks[blockIdx.x];
   if (b.sim < 0) return;  // padding of the XCD order (block-uniform)
   const RollDesc& S = sims[b.sim];
-  adj_pre_tile(roll_state(S, seg, k, 2), S.D, S.Dx, S.Dy, S.lam + 2 * S.cells,
-               roll_inject(S, j, fe), S.H, S.W, S.tab, S.pre[0], b.bx, b.by, tab);
+  adj_pre_tile(roll_state(S, seg, k, 2), gfield(S.D), gfield(S.Dx), gfield(S.Dy),
+               gfield(S.lam) + 2 * S.cells, roll_inject(S, j, fe), S.H, S.W, gfield(S.tab),
+               gfield(S.pre[0]), b.bx, b.by, tab);
 }
 
 // Reverse of step c + k of segment `seg`: cotangent set `cur` (0 lam, 1
@@ -195,16 +213,16 @@ k_roll_main(const RollDesc* __restrict__ sims, const RollBlock* __restrict__ blo
   if (b.sim < 0) return;  // padding of the XCD order (block-uniform)
   const RollDesc& S = sims[b.sim];
   const size_t n = S.cells;
-  float* lin = cur ? S.lam_alt : S.lam;
-  float* lout = cur ? S.lam : S.lam_alt;
+  float* lin = cur ? gfield(S.lam_alt) : gfield(S.lam);
+  float* lout = cur ? gfield(S.lam) : gfield(S.lam_alt);
   const AdjNext nx{NEXT ? roll_state(S, seg, k - 1, 2) : nullptr,
-                   NEXT ? roll_inject(S, c0 + k, fe) : nullptr, S.D, S.Dx, S.Dy,
-                   S.pre[1 - set]};
+                   NEXT ? roll_inject(S, c0 + k, fe) : nullptr, gfield(S.D), gfield(S.Dx),
+                   gfield(S.Dy), gfield(S.pre[1 - set])};
   const BatchStimView sv{S.stim, S.n_stim, masks[b.sim]};
   adj_main_tile<NEXT>(S.c, roll_state(S, seg, k, 0), roll_state(S, seg, k, 1),
                       roll_state(S, seg, k, 2), S.H, S.W, sv, S.dt, S.inv_dx2, lin, lin + n,
-                      S.pre[set], lout, lout + n, lout + 2 * n, S.gD, S.part, S.tab, nx, b.bx,
-                      b.by, S.gx, tab, xch);
+                      gfield(S.pre[set]), lout, lout + n, lout + 2 * n, gfield(S.gD),
+                      gfield(S.part), gfield(S.tab), nx, b.bx, b.by, S.gx, tab, xch);
 }
 
 }  // namespace fk
@@ -225,6 +243,7 @@ struct fk_rollout_batch {
   uint32_t* d_masks = nullptr;  // n_steps x n
   int n_tiles = 0, n_pre = 0, n_main = 0;
   size_t max_cells = 0;
+  int traj_seg = -1;  // segment whose states the trajectory buffer holds (-1: none)
 };
 
 // Work-list order for the chip's placement of workgroup w on XCD w % 8 (each
@@ -462,6 +481,7 @@ int32_t fk_rollout_batch_forward(fk_rollout_batch* b, void* stream) {
                                      seg + 1 < b->n_seg ? 1 : 0);
   }
   HIP_CHECK(hipGetLastError());
+  b->traj_seg = b->n_seg - 1;
   return FK_OK;
 }
 
@@ -480,9 +500,13 @@ int32_t fk_rollout_batch_backward(fk_rollout_batch* b, float* g_tau, void* strea
   const int step_blocks = (b->n_tiles + 3) / 4;
   for (int seg = b->n_seg - 1; seg >= 0; --seg) {
     const int c = seg * b->S, len = std::min(b->S, b->n_steps - c);
-    for (int k = 0; k + 1 < len; ++k)  // recompute x_{c+1} .. x_{c+len-1}
-      k_roll_step<kRollRows><<<step_blocks, 256, 0, s>>>(
-          b->d_sims, b->d_tiles, b->n_tiles, b->d_masks + (size_t)(c + k) * b->n, seg, k);
+    // recompute x_{c+1} .. x_{c+len-1}, unless the forward's last segment is
+    // still in place (the same states: same kernels from the same checkpoint)
+    if (seg != b->traj_seg)
+      for (int k = 0; k + 1 < len; ++k)
+        k_roll_step<kRollRows><<<step_blocks, 256, 0, s>>>(
+            b->d_sims, b->d_tiles, b->n_tiles, b->d_masks + (size_t)(c + k) * b->n, seg, k);
+    b->traj_seg = seg;
     k_roll_pre<<<b->n_pre, kAdjThreads, 0, s>>>(b->d_sims, b->d_pre, seg, len - 1, c + len,
                                                 b->fe);
     int cur = 0, set = 0;

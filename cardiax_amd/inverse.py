@@ -98,12 +98,20 @@ class _Param(NamedTuple):
         return self.Ry @ self.theta @ self.Rx.t()
 
 
+# Device memory the batched plan may spend on keeping every state of a rollout
+# (no checkpoint recompute); above it, states are recomputed from sqrt(N)
+# checkpoints. The gradients are the same bits either way.
+TRAJECTORY_BUDGET_BYTES = 64 << 30
+
+
 def _plan(instances: Sequence[Instance], dt: float, dx: float) -> adjoint.RolloutBatch:
     n_steps, fe = instances[0].n_steps, instances[0].frame_every
     if any(x.n_steps != n_steps or x.frame_every != fe for x in instances):
         raise ValueError("a batched plan needs one n_steps and frame_every for all instances")
+    cells = sum(int(x.D_true.numel()) for x in instances)
+    full = 12 * cells * n_steps <= TRAJECTORY_BUDGET_BYTES
     return adjoint.RolloutBatch([(x.state0, x.params, x.stimuli, 0) for x in instances],
-                                n_steps, fe, dt, dx)
+                                n_steps, fe, dt, dx, checkpoint_every=n_steps if full else None)
 
 
 def _frame_losses(frames, instances) -> List[torch.Tensor]:
