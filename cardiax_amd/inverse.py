@@ -125,16 +125,17 @@ def _frame_losses(frames, instances) -> List[torch.Tensor]:
 
 def recover(instances: Sequence[Instance], iters: int = 50, lr: float = 2e-5,
             n_streams: int = 8, coarse: int = 8, dt: float = wl.DT,
-            dx: float = wl.DX, batched: bool = True) -> dict:
+            dx: float = wl.DX, batched: bool = True, d_err_at=()) -> dict:
     """Runs ``iters`` Adam iterations of forward + adjoint on every instance.
     Returns the estimates and per-instance losses and D errors (relative L2
     distance to the hidden map, before and after), plus the wall time.
     ``batched``: all instances through one RolloutBatch plan (they must share
     n_steps and frame_every); otherwise per-instance rollouts on
-    ``n_streams`` streams."""
+    ``n_streams`` streams. ``d_err_at``: iteration counts after which the
+    batched path also records every instance's D error (``res["d_err_at"]``)."""
     dev = require_gpu("inverse")
     if batched:
-        return _recover_batched(instances, iters, lr, coarse, dt, dx, dev)
+        return _recover_batched(instances, iters, lr, coarse, dt, dx, dev, tuple(d_err_at))
     streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
     params = _init_params(instances, coarse, dev)
     opts = [torch.optim.Adam([prm.theta], lr=lr) for prm in params]
@@ -166,8 +167,9 @@ def _init_params(instances, coarse, dev):
     return params
 
 
-def _recover_batched(instances, iters, lr, coarse, dt, dx, dev) -> dict:
+def _recover_batched(instances, iters, lr, coarse, dt, dx, dev, d_err_at=()) -> dict:
     params = _init_params(instances, coarse, dev)
+    marks = {}
     opts = [torch.optim.Adam([prm.theta], lr=lr) for prm in params]
     losses = torch.zeros((len(instances), iters), dtype=torch.float64, device=dev)
     with _plan(instances, dt, dx) as plan:
@@ -185,9 +187,19 @@ def _recover_batched(instances, iters, lr, coarse, dt, dx, dev) -> dict:
                 with torch.no_grad():  # interpolation keeps D within the nodes' range
                     prm.theta.clamp_(D_MIN, D_MAX)
                 losses[k, it] = ls[k].detach()
+            if it + 1 in d_err_at:  # D errors after this iteration (on the device)
+                marks[it + 1] = [_d_err(inst, prm) for inst, prm in zip(instances, params)]
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-    return _result(instances, params, losses, wall)
+    res = _result(instances, params, losses, wall)
+    res["d_err_at"] = {k: [float(x) for x in v] for k, v in marks.items()}
+    return res
+
+
+def _d_err(inst, prm) -> torch.Tensor:
+    with torch.no_grad():
+        ref = inst.D_true.double()
+        return torch.linalg.norm(prm.D().double() - ref) / torch.linalg.norm(ref)
 
 
 def _result(instances, params, losses, wall) -> dict:
