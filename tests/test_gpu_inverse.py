@@ -99,3 +99,44 @@ def test_batched_recover_matches_per_instance_recover(gpu):
     for x, y in zip(a["D"], b["D"]):
         assert torch.equal(x, y)
     np.testing.assert_array_equal(a["loss"], b["loss"])
+
+
+def test_rollout_batch_mixed_problems_equal_per_problem(gpu):
+    """Problems with different parameter sets, grids, start times and stimuli
+    (a periodic one included) in one plan: frames and dL/dD bitwise those of
+    each problem's own rollout, with the default sqrt(N) checkpoints."""
+    from cardiax_amd import params as P
+    from cardiax_amd import solve
+    from cardiax_amd.stimulus import Protocol
+
+    rng = np.random.default_rng(11)
+    names = ["PARAMSET_1A", "PARAMSET_3", "PARAMSET_4A", "PARAMSET_7"]
+    shapes = [(48, 40), (64, 64), (37, 90), (96, 80)]
+    probs, Ds = [], []
+    for k, (name, sh) in enumerate(zip(names, shapes)):
+        dev = torch.device("cuda", torch.cuda.current_device())
+        f1 = torch.zeros(sh, device=dev)
+        f1[:, : max(1, sh[1] // 8)] = 20.0
+        f2 = torch.zeros(sh, device=dev)
+        f2[sh[0] // 3: sh[0] // 2, 3: sh[1] // 2] = 20.0
+        stims = [(Protocol(0, 2, 1e9), f1), (Protocol(17 + 3 * k, 2, 60), f2)]
+        st = solve.init(sh)
+        t0 = 5 * k
+        probs.append((st, getattr(P, name), stims, t0))
+        nodes = inverse.D_MIN + (inverse.D_MAX - inverse.D_MIN) * rng.uniform(size=(4, 4))
+        Ry, Rx = inverse.interp_matrix(sh[0], 4, dev), inverse.interp_matrix(sh[1], 4, dev)
+        Ds.append((Ry @ torch.tensor(nodes, dtype=torch.float32, device=dev) @ Rx.t())
+                  .contiguous())
+    n_steps, fe = 240, 30
+    with adjoint.RolloutBatch(probs, n_steps, fe, 0.01, 0.01) as plan:
+        frames = [f.clone() for f in plan.forward(Ds)]
+        g = [torch.randn_like(f) for f in frames]
+        grads = [x.clone() for x in plan.backward(g)]
+    torch.cuda.synchronize()
+    for k, (st, prm, stims, t0) in enumerate(probs):
+        D = Ds[k].clone().requires_grad_(True)
+        fr = adjoint.rollout(D, prm.tau_d, prm.tau_r, prm.tau_si, st, prm, stims, t0, n_steps,
+                             fe, 0.01, 0.01)
+        assert torch.isfinite(fr).all() and torch.equal(fr, frames[k])
+        (fr * g[k]).sum().backward()
+        assert torch.equal(D.grad, grads[k])
