@@ -202,14 +202,22 @@ def bench_batched(n_sims, n_steps, seed, per_sim_steps=64):
                                      "%d simulations, %.1f M cells" % (n_sims, cells / 1e6))}
 
 
+def traffic_rate(traffic, seconds_per_launch):
+    """Measured bytes per launch over this run's time per launch, GB/s (None
+    without a measurement): the rate beside the algorithmic one."""
+    return None if traffic is None else round(traffic / seconds_per_launch / 1e9, 1)
+
+
 def leg_roofline(bytes_per_launch, seconds_per_launch, kernel, key):
     """HBM roofline of a leg's dominant kernel: algorithmic bytes per launch
     over the measured time per launch, with the traffic a committed
     rocprofv3 --pmc summary measured for it (None when there is none)."""
     achieved = bytes_per_launch / seconds_per_launch / 1e9
+    traffic = measured_traffic_key("%s %s" % (kernel, key))
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic_key("%s %s" % (kernel, key)), "kernel": kernel,
+            "traffic": traffic, "traffic_gbs": traffic_rate(traffic, seconds_per_launch),
+            "kernel": kernel,
             "per_launch_us": round(seconds_per_launch * 1e6, 3),
             "bytes_per_launch": int(bytes_per_launch), "traffic_counts": TRAFFIC_NOTE}
 
@@ -722,7 +730,9 @@ def main():
     achieved = BYTES_PER_CELL_UPDATE * H * W * depth / per_launch_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(kernel, H, W), "kernel": kernel,
+                "traffic": measured_traffic(kernel, H, W),
+                "traffic_gbs": traffic_rate(measured_traffic(kernel, H, W), per_launch_s),
+                "kernel": kernel,
                 "per_launch_us": round(per_launch_s * 1e6, 3),
                 "bytes_per_launch": BYTES_PER_CELL_UPDATE * H * W * depth,
                 "traffic_counts": TRAFFIC_NOTE}
