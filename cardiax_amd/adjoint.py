@@ -173,7 +173,8 @@ class RolloutBatch:
     ``problems``: (state0, params, stimuli, t0) per problem; the grids may
     differ in size. The plan owns the device buffers: ``D`` (one tensor per
     problem, overwritten by ``forward``), ``frames``, ``g_frames`` and
-    ``grad_D``. Use it as a context manager or call ``close``."""
+    ``grad_D``. Use it as a context manager or call ``close``. A plan runs on
+    the current stream of each call; calls must not overlap."""
 
     def __init__(self, problems, n_steps: int, frame_every: int, dt: float, dx: float,
                  checkpoint_every: int | None = None):

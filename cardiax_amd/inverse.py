@@ -109,9 +109,13 @@ def _plan(instances: Sequence[Instance], dt: float, dx: float) -> adjoint.Rollou
     if any(x.n_steps != n_steps or x.frame_every != fe for x in instances):
         raise ValueError("a batched plan needs one n_steps and frame_every for all instances")
     cells = sum(int(x.D_true.numel()) for x in instances)
-    full = 12 * cells * n_steps <= TRAJECTORY_BUDGET_BYTES
-    return adjoint.RolloutBatch([(x.state0, x.params, x.stimuli, 0) for x in instances],
-                                n_steps, fe, dt, dx, checkpoint_every=n_steps if full else None)
+    problems = [(x.state0, x.params, x.stimuli, 0) for x in instances]
+    if 12 * cells * n_steps <= TRAJECTORY_BUDGET_BYTES:
+        try:
+            return adjoint.RolloutBatch(problems, n_steps, fe, dt, dx, checkpoint_every=n_steps)
+        except RuntimeError:  # the device cannot hold every state: checkpoint instead
+            pass
+    return adjoint.RolloutBatch(problems, n_steps, fe, dt, dx)
 
 
 def _frame_losses(frames, instances) -> List[torch.Tensor]:
