@@ -246,36 +246,6 @@ struct fk_rollout_batch {
   int traj_seg = -1;  // segment whose states the trajectory buffer holds (-1: none)
 };
 
-// Work-list order for the chip's placement of workgroup w on XCD w % 8 (each
-// XCD has its own L2): `runs` are groups of consecutive workgroups that share
-// halo rows (a column of tiles); each run goes whole to the XCD with the least
-// work so far, and the XCD lists are interleaved, short ones padded with
-// `pad` items that exit at once. `per_wg` items make one workgroup.
-constexpr int kXcds = 8;
-
-template <class T>
-static std::vector<T> xcd_order(const std::vector<std::vector<T>>& runs, int per_wg, T pad) {
-  std::vector<std::vector<T>> lists(kXcds);
-  for (const auto& r : runs) {
-    size_t best = 0;
-    for (int x = 1; x < kXcds; ++x)
-      if (lists[x].size() < lists[best].size()) best = x;
-    lists[best].insert(lists[best].end(), r.begin(), r.end());
-  }
-  size_t m = 0;
-  for (auto& l : lists) {
-    while (l.size() % per_wg) l.push_back(pad);
-    m = std::max(m, l.size() / per_wg);
-  }
-  std::vector<T> out;
-  out.reserve(m * kXcds * per_wg);
-  for (size_t j = 0; j < m; ++j)
-    for (int x = 0; x < kXcds; ++x)
-      for (int k = 0; k < per_wg; ++k)
-        out.push_back(j * per_wg + k < lists[x].size() ? lists[x][j * per_wg + k] : pad);
-  return out;
-}
-
 static int32_t roll_check(const fk_rollout_spec& s, int i) {
   if (!check_params(&s.params) || !check_grid(s.H, s.W) ||
       !check_ptrs({s.D, s.v0, s.w0, s.u0, s.frames, s.g_frames, s.grad_D}) ||
