@@ -116,3 +116,25 @@ def test_random_diffusivity_domain_and_mask(gpu):
     assert D.min() == np.float32(2e-4) and D.max() == np.float32(5e-4)
     m = scars.random_spline(11, (96, 96)).cpu().numpy()
     assert set(np.unique(m)) == {0.0, 1.0}
+
+
+def test_scar_outside_the_image_becomes_the_centre_pixel(gpu):
+    """On tall images the reference's row mapping (rows = shape[1] - y) can put
+    the whole scar outside; the mask of one value then becomes one pixel at the
+    centre, and the D-map is the blur of that pixel, rescaled."""
+    shape = (150, 400)
+    seed = 12
+    g1, g2, size, sigma = scars._blur_draws(seed, shape)
+    cs = scars.blob_contours(g2, shape, scars.DEF_PARAMS, scars.spline_centroids(g1))
+    assert not so.union_mask(cs, shape).any()  # the whole scar falls outside
+    h1, h2, _, _ = scars._blur_draws(seed, shape)  # the same draws again
+    m = scars.random_spline(h2, shape, scars.DEF_PARAMS, scars.spline_centroids(h1))
+    m = m.cpu().numpy()
+    want = np.zeros(shape, np.float32)
+    want[75, 200] = 1.0
+    assert np.array_equal(m, want)
+    D = generate.random_diffusivity(seed, shape).cpu().numpy()
+    blurred = so.convolve_same(want, so.gauss_kernel(size, sigma)).astype(np.float32)
+    ref = so.scar_to_diffusivity(blurred)
+    assert D.min() == np.float32(1e-4) and D.max() == np.float32(1e-3)
+    np.testing.assert_allclose(D, ref, rtol=0, atol=1e-9)
