@@ -771,7 +771,7 @@ using namespace fk;
 
 extern "C" {
 
-int32_t fk_abi_version(void) { return 2; }
+int32_t fk_abi_version(void) { return 3; }
 
 int32_t fk_set_forward_variant(int32_t variant) {
   ErrorScope es;
