@@ -2,8 +2,8 @@
 
 References: the FP64 torch-autograd restatement of the reference step
 (oracle/fk_autograd.py, the stand-in for jax.grad) and central finite
-differences of the FP64 oracle. Tolerance: 1e-3 relative (the north-star bar
-for the D-map gradient); single steps are checked tighter.
+differences of the FP64 oracle. Tolerance: 1e-3 relative for the D-map
+gradient of a rollout; single steps are checked tighter.
 """
 import ctypes
 
