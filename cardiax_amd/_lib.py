@@ -135,7 +135,7 @@ _SIGNATURES = {
                                       c_sz, c_vp]),
     "fk_convolve_same": (c_i32, [c_vp, c_i32, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_sz,
                                  c_vp]),
-    "fk_convolve_workspace_bytes": (c_sz, [c_i32, c_i32]),
+    "fk_convolve_workspace_bytes": (c_sz, [c_i32, c_i32, c_i32, c_i32]),
     "fk_scar_to_diffusivity": (c_i32, [c_vp, c_i64, c_d, c_d, c_vp, c_vp, c_sz, c_vp]),
     "fk_batch_create": (c_i32, [ctypes.POINTER(FkSim), c_i32, ctypes.POINTER(c_vp)]),
     "fk_rollout_batch_create": (c_i32, [ctypes.POINTER(FkRolloutSpec), c_i32, c_i32, c_i32,

@@ -25,6 +25,7 @@
 
 #include "cardiax_amd.h"
 #include "fk_host.h"
+#include "fk_stream.h"
 
 namespace fk {
 
@@ -72,13 +73,29 @@ k_raster_polygons(float* __restrict__ mask, int H, int W, const double* __restri
 // by up to R outputs; the chunk's rows stay in L1 across q) and the chunk's
 // 2R - 1 taps of column q, contiguous and wave-uniform in the padded,
 // transposed copy of the kernel (k_pad_kernel: zeros outside [0, kh), so no
-// test per tap). Out-of-image inputs are 0; input loads are clamped into the
-// image and the value selected away. Per output the sum runs over chunks,
-// then q, then d, in double; rounded once.
+// test per tap). The inputs come from a zero-padded copy of the image
+// (k_pad_image) wide enough for every row and column a thread reaches, so
+// the loads need no clamp and no select. Per output the sum runs over
+// chunks, then q, then d, in double; rounded once.
 constexpr int kConvRows = 8;
 constexpr int kConvPadFront = 2 * kConvRows - 2;
 
 __host__ __device__ constexpr int conv_padded_rows(int kh) { return kh + 3 * kConvRows - 3; }
+
+// The zero-padded image: PT rows above and below, PL columns left and PR
+// right of the H x W image.
+struct ConvPad {
+  int PT, PL, Hp, Wp;
+};
+
+__host__ __device__ inline ConvPad conv_pad(int H, int W, int kh, int kw) {
+  ConvPad p;
+  p.PT = kh + 2 * kConvRows;
+  p.PL = kw;
+  p.Hp = H + 2 * p.PT;
+  p.Wp = W + 2 * kw + 64;
+  return p;
+}
 
 __global__ void __launch_bounds__(256)
 k_pad_kernel(const double* __restrict__ k, int kh, int kw, double* __restrict__ kt) {
@@ -90,8 +107,18 @@ k_pad_kernel(const double* __restrict__ k, int kh, int kw, double* __restrict__ 
 }
 
 __global__ void __launch_bounds__(256)
-k_convolve_same(const float* __restrict__ src, int H, int W, const double* __restrict__ kt,
-                int kh, int kw, float* __restrict__ dst) {
+k_pad_image(const float* __restrict__ src, int H, int W, ConvPad pd, float* __restrict__ pad) {
+  const size_t n = (size_t)pd.Hp * pd.Wp;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n;
+       idx += (size_t)gridDim.x * 256) {
+    const int r = (int)(idx / pd.Wp) - pd.PT, c = (int)(idx % pd.Wp) - pd.PL;
+    pad[idx] = (r >= 0 && r < H && c >= 0 && c < W) ? src[(size_t)r * W + c] : 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_convolve_same(const float* __restrict__ pad, int H, int W, ConvPad pd,
+                const double* __restrict__ kt, int kh, int kw, float* __restrict__ dst) {
   constexpr int R = kConvRows;
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * R);
@@ -104,26 +131,16 @@ k_convolve_same(const float* __restrict__ src, int H, int W, const double* __res
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
   for (int ch = 0; ch < n_chunks; ++ch) {
     const int d0 = -(kh - 1) + ch * R;
-    const float* rows[R];
-    bool row_in[R];
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const int si = i0 + oi + d0 + m;  // wave-uniform
-      row_in[m] = si >= 0 && si < H;
-      rows[m] = src + (size_t)min(max(si, 0), H - 1) * W;
-    }
+    // padded row of input row i0 + oi + d0 + m (wave-uniform)
+    const float* row0 = pad + (size_t)(i0 + oi + d0 + pd.PT) * pd.Wp;
     // taps p = -d0 - (R-1) + e of column q sit at kt[q kp + R - 1 - d0 + e]
     const double* tcol = kt + (R - 1 - d0);
     for (int q = 0; q < kw; ++q) {
-      const int c = j + oj - q;
-      const bool c_in = c >= 0 && c < W;
-      const int cc = min(max(c, 0), W - 1);
+      const uint32_t cb = (uint32_t)(j + oj - q + pd.PL) * 4u;  // padded column, bytes
       double x[R];
 #pragma unroll
-      for (int m = 0; m < R; ++m) {
-        const float v = rows[m][cc];
-        x[m] = (row_in[m] && c_in) ? (double)v : 0.0;
-      }
+      for (int m = 0; m < R; ++m)
+        x[m] = (double)ld(row0 + (size_t)m * pd.Wp, cb);
       const double* t = tcol + (size_t)q * kp;
 #pragma unroll
       for (int m = 0; m < R; ++m)    // d = d0 + m
@@ -280,9 +297,11 @@ size_t fk_rasterize_workspace_bytes(int32_t n_polys, int32_t n_vertices) {
          (size_t)n_polys * sizeof(PolyBox);
 }
 
-size_t fk_convolve_workspace_bytes(int32_t kh, int32_t kw) {
-  if (kh < 1 || kw < 1) return 0;
-  return (size_t)kw * conv_padded_rows(kh) * sizeof(double);
+size_t fk_convolve_workspace_bytes(int32_t H, int32_t W, int32_t kh, int32_t kw) {
+  if (H < 1 || W < 1 || kh < 1 || kw < 1) return 0;
+  const ConvPad pd = conv_pad(H, W, kh, kw);
+  const size_t taps = ((size_t)kw * conv_padded_rows(kh) * sizeof(double) + 255) / 256 * 256;
+  return taps + (size_t)pd.Hp * pd.Wp * sizeof(float);
 }
 
 int32_t fk_convolve_same(const float* src, int32_t H, int32_t W, const double* kernel,
@@ -292,19 +311,26 @@ int32_t fk_convolve_same(const float* src, int32_t H, int32_t W, const double* k
   if (!check_ptrs({src, kernel, dst, work})) return last_code();
   if (H < 1 || W < 1 || kh < 1 || kw < 1)
     return set_error(FK_ERR_ARG, "fk_convolve_same: empty image or kernel");
-  if (kh > (1 << 20) || kw > (1 << 20))
-    return set_error(FK_ERR_ARG, "fk_convolve_same: kernel of %d x %d is too large", kh, kw);
+  if (kh > (1 << 16) || kw > (1 << 16) || H > (1 << 20) || W > (1 << 20))
+    return set_error(FK_ERR_ARG, "fk_convolve_same: image %d x %d or kernel %d x %d too large",
+                     H, W, kh, kw);
   if (src == dst) return set_error(FK_ERR_ARG, "fk_convolve_same: dst must not be src");
-  const size_t need = fk_convolve_workspace_bytes(kh, kw);
+  const size_t need = fk_convolve_workspace_bytes(H, W, kh, kw);
   if (work_bytes < need)
     return set_error(FK_ERR_ARG, "workspace too small: need %zu bytes, got %zu", need,
                      work_bytes);
   hipStream_t s = (hipStream_t)stream;
   double* kt = (double*)work;
+  const size_t taps = ((size_t)kw * conv_padded_rows(kh) * sizeof(double) + 255) / 256 * 256;
+  float* pad = (float*)((char*)work + taps);
+  const ConvPad pd = conv_pad(H, W, kh, kw);
   const int n_pad = kw * conv_padded_rows(kh);
   k_pad_kernel<<<(n_pad + 255) / 256, 256, 0, s>>>(kernel, kh, kw, kt);
+  const size_t n_img = (size_t)pd.Hp * pd.Wp;
+  k_pad_image<<<(int)std::min<size_t>((n_img + 255) / 256, 4096), 256, 0, s>>>(src, H, W, pd,
+                                                                              pad);
   const dim3 grid((W + 63) / 64, (H + 4 * kConvRows - 1) / (4 * kConvRows));
-  k_convolve_same<<<grid, 256, 0, s>>>(src, H, W, kt, kh, kw, dst);
+  k_convolve_same<<<grid, 256, 0, s>>>(pad, H, W, pd, kt, kh, kw, dst);
   HIP_CHECK(hipGetLastError());
   return FK_OK;
 }

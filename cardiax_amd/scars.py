@@ -187,7 +187,8 @@ def convolve_same(img: torch.Tensor, kernel: np.ndarray) -> torch.Tensor:
     kd = torch.from_numpy(k).to(src.device)
     dst = torch.empty_like(src)
     L = _lib.lib()
-    nbytes = int(L.fk_convolve_workspace_bytes(k.shape[0], k.shape[1]))
+    nbytes = int(L.fk_convolve_workspace_bytes(src.shape[0], src.shape[1], k.shape[0],
+                                               k.shape[1]))
     work = torch.empty(max(1, nbytes), dtype=torch.uint8, device=src.device)
     _lib.check(L.fk_convolve_same(_lib.ptr(src), src.shape[0], src.shape[1], _lib.ptr(kd),
                                   k.shape[0], k.shape[1], _lib.ptr(dst), _lib.ptr(work), nbytes,

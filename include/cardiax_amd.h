@@ -499,14 +499,15 @@ int32_t fk_rasterize_polygons(float* mask, int32_t H, int32_t W, const double* r
  * dst[i, j] = sum_{p, q} kernel[p, q] src[i + (kh-1)/2 - p, j + (kw-1)/2 - q]
  * with integer halves and zero outside the image; products and sums in
  * double, rounded once to float. dst must not be src. work:
- * fk_convolve_workspace_bytes(kh, kw) of device memory. Stream-ordered. */
+ * fk_convolve_workspace_bytes(H, W, kh, kw) of device memory. Stream-ordered. */
 int32_t fk_convolve_same(const float* src, int32_t H, int32_t W, const double* kernel,
                          int32_t kh, int32_t kw, float* dst, void* work, size_t work_bytes,
                          void* stream);
 
-/* Device scratch of fk_convolve_same for a kh x kw kernel (its zero-padded,
- * transposed copy). */
-size_t fk_convolve_workspace_bytes(int32_t kh, int32_t kw);
+/* Device scratch of fk_convolve_same for an H x W image and a kh x kw kernel
+ * (the kernel's zero-padded, transposed copy and a zero-padded copy of the
+ * image). */
+size_t fk_convolve_workspace_bytes(int32_t H, int32_t W, int32_t kh, int32_t kw);
 
 /* deepx/utils_scars.py:426 with deepx/generate.py:79-83 and
  * cardiax/convert.py:23-26: D = diffusivity_rescale(1 - scar, (lo, hi)), i.e.
