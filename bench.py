@@ -399,7 +399,12 @@ def bench_inverse(seed, n=32, iters=100, n_steps=1000, frame_every=50, per_insta
                                           if 50 in res["d_err_at"] else None),
             "d_err_per_instance": [[int(x.D_true.shape[0]), round(float(a), 4), round(float(b), 4)]
                                    for x, a, b in zip(inst, d0, d1)],
-            "instances_improved": int((d1 < d0).sum())}
+            "instances_improved": int((d1 < d0).sum()),
+            # the same over the cells a target frame shows excited (the cells the
+            # frames constrain)
+            "reached_frac_mean": round(float(np.mean(res["reached_frac"])), 4),
+            "d_err_reached_initial": round(float(np.mean(res["d_err_reached_initial"])), 4),
+            "d_err_reached_final": round(float(np.mean(res["d_err_reached_final"])), 4)}
 
 
 def bench_heun(seed, size=1024, n_steps=2000):

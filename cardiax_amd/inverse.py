@@ -206,18 +206,32 @@ def _d_err(inst, prm) -> torch.Tensor:
         return torch.linalg.norm(prm.D().double() - ref) / torch.linalg.norm(ref)
 
 
+def reached(inst: Instance, threshold: float = 0.5) -> torch.Tensor:
+    """Cells some target frame shows excited (u >= threshold): where a wave
+    passed while the frames were sampled, the only cells whose D the frames
+    constrain."""
+    return inst.target.max(dim=0).values >= threshold
+
+
 def _result(instances, params, losses, wall) -> dict:
-    err0, err1, Ds = [], [], []
+    err0, err1, act0, act1, frac, Ds = [], [], [], [], [], []
     for inst, prm in zip(instances, params):
         with torch.no_grad():
             D = prm.D()
-        Ds.append(D)
-        ref = inst.D_true.double()
-        nrm = float(torch.linalg.norm(ref))
-        err0.append(float(torch.linalg.norm(torch.full_like(ref, D_MAX) - ref)) / nrm)
-        err1.append(float(torch.linalg.norm(D.double() - ref)) / nrm)
+            Ds.append(D)
+            ref = inst.D_true.double()
+            D0 = torch.full_like(ref, D_MAX)
+            nrm = float(torch.linalg.norm(ref))
+            err0.append(float(torch.linalg.norm(D0 - ref)) / nrm)
+            err1.append(float(torch.linalg.norm(D.double() - ref)) / nrm)
+            m = reached(inst)
+            frac.append(float(m.double().mean()))
+            nm = float(torch.linalg.norm(ref[m])) if bool(m.any()) else 0.0
+            act0.append(float(torch.linalg.norm((D0 - ref)[m])) / nm if nm > 0 else 0.0)
+            act1.append(float(torch.linalg.norm((D.double() - ref)[m])) / nm if nm > 0 else 0.0)
     return {"D": Ds, "loss": losses.cpu().numpy(), "seconds": wall,
-            "d_err_initial": err0, "d_err_final": err1}
+            "d_err_initial": err0, "d_err_final": err1,
+            "d_err_reached_initial": act0, "d_err_reached_final": act1, "reached_frac": frac}
 
 
 def _loss_backward(inst: Instance, D: torch.Tensor, dt: float, dx: float) -> torch.Tensor:
