@@ -229,6 +229,13 @@ class RolloutBatch:
         """Frames of every problem at the D-maps ``Ds`` (copied into the plan's
         D buffers): the plan's ``frames`` tensors, overwritten."""
         self._require_open()
+        Ds = list(Ds)
+        if len(Ds) != self.n:
+            raise ValueError("%d D-maps for %d problems" % (len(Ds), self.n))
+        for src, dst in zip(Ds, self.D):
+            if tuple(src.shape) != tuple(dst.shape):
+                raise ValueError("D has shape %s, the problem's grid is %s"
+                                 % (tuple(src.shape), tuple(dst.shape)))
         with torch.no_grad():
             for dst, src in zip(self.D, Ds):
                 dst.copy_(src)
@@ -242,6 +249,10 @@ class RolloutBatch:
         gradients with respect to (tau_d, tau_r, tau_si)."""
         self._require_open()
         if g_frames is not None:
+            g_frames = list(g_frames)
+            if len(g_frames) != self.n or any(tuple(g.shape) != tuple(f.shape)
+                                              for g, f in zip(g_frames, self.g_frames)):
+                raise ValueError("g_frames must match the plan's frames in number and shape")
             with torch.no_grad():
                 for dst, src in zip(self.g_frames, g_frames):
                     dst.copy_(src)
